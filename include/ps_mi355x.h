@@ -1,0 +1,190 @@
+/* ps_mi355x -- C ABI of the MI355X period scan.
+ *
+ * The drop-in boundary for the reference's GPU layer.  The reference exposes
+ * four C++ functions (start_CUDA.h:1-18) that read most of their inputs from
+ * process globals; this ABI passes everything explicitly, returns int status
+ * codes (never exits the process) and writes into caller-owned buffers.
+ *
+ *   reference                                    this ABI
+ *   CUDAPrepare   (start_CUDA.cu:207-352)  ->    ps_prepare
+ *   CUDAPrecalc   (start_CUDA.cu:400-789)  ->    ps_precalc (one conw round)
+ *   CUDAStart     (start_CUDA.cu:792-1245) ->    ps_start
+ *   CUDAUnprepare (start_CUDA.cu:356-373)  ->    ps_unprepare
+ *
+ * The application-level pieces the reference keeps in main()
+ * (period_search_BOINC.cpp:343-866: input parsing, geometry tables, the
+ * convexity-weight loop) are exported too, so a caller can either hand over
+ * its own tables (ps_problem) or let the library read period_search_in.
+ *
+ * All arithmetic is IEEE FP64.  Functions are synchronous; the library owns
+ * its HIP stream and device memory.  Status codes: PS_OK or a negative
+ * PS_E_* value; ps_last_error() describes the last failure on a context.
+ */
+#ifndef PS_MI355X_H
+#define PS_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS_OK 0
+#define PS_E_ARG (-1)         /* bad argument / size limit */
+#define PS_E_INPUT (-2)       /* malformed period_search_in */
+#define PS_E_HIP (-3)         /* HIP runtime failure */
+#define PS_E_UNSUPPORTED (-4) /* problem outside what the kernels support */
+#define PS_E_STATE (-5)       /* call order (e.g. ps_start before ps_prepare) */
+
+#define PS_N_POLES 10
+
+typedef struct ps_ctx ps_ctx;
+
+/* Everything the scan reads.  Arrays are 0-based; N = ndata (data points
+ * only, the three convexity rows are added by the library), L = n_curves,
+ * F = nfac, C = ncoef, ma = C + 8.  The struct mirrors the globals and
+ * arguments of CUDAPrepare/CUDAPrecalc/CUDAStart (globals.h:6-22,
+ * start_CUDA.h:1-18). */
+typedef struct {
+  /* observations */
+  int ndata, n_curves;
+  const int* l_points;     /* [L] */
+  const int* in_rel;       /* [L], 1 = relative curve */
+  const double* tim;       /* [N] days minus the epoch */
+  const double* ee;        /* [3*N] asteroid->Earth unit vectors, component-major */
+  const double* ee0;       /* [3*N] asteroid->Sun unit vectors, component-major */
+  const double* brightness;/* [N] */
+  const double* sig;       /* [N] curve mean brightness */
+  const double* sigr2;     /* [N] 1/sig^2 */
+  const double* weight;    /* [N] */
+  /* shape basis */
+  int nfac, lmax, mmax, ncoef;
+  const double* normal;    /* [3*F] unit facet normals, component-major */
+  const double* d_area;    /* [F] */
+  const double* pleg;      /* [(mmax+1)*(lmax+1)*F] P_l^m at facet normals */
+  const double* fc;        /* [(mmax+1)*F] cos(m phi) */
+  const double* fs;        /* [(mmax+1)*F] sin(m phi) */
+  const double* d_sphere;  /* [C*F] basis row k-1 = d_sphere[k] */
+  const double* cg_first;  /* [C] initial ellipsoid coefficients */
+  const int* ia;           /* [ma] free (1) / fixed (0), ia[i] is parameter i+1 */
+  /* model and LM controls */
+  double par[3];           /* phase function A, D, k */
+  double cl;               /* initial Lambert weight (not its log) */
+  double phi0;             /* radians */
+  double alamda_start, alamda_incr, alamda_incrr;
+  double stop_condition;
+  int n_iter_min;
+  /* scan */
+  double f_start, f_end, f_step;   /* 1/hours */
+  double escl;                     /* initial size scale, for the first line */
+  double beta_pole[PS_N_POLES], lambda_pole[PS_N_POLES]; /* degrees */
+} ps_problem;
+
+/* One output line of period_search_out (start_CUDA.cu:1222-1226). */
+typedef struct {
+  double period_h, rms, chi2, dark_or_conw, lambda_deg, beta_deg;
+} ps_line;
+
+/* Per-pair result of one LM fit (Start.cu:5056-5125), plus the number of
+ * visible (point, facet) pairs the fit evaluated (its algorithmic work). */
+typedef struct {
+  int reported, niter;
+  double dev, dark, period, lambda_deg, beta_deg, chisq;
+  long long n_vis;
+} ps_pair_result;
+
+/* ---- host-only helpers (no device needed) ---- */
+
+/* The parsed period_search_in with its geometry tables, as the reference's
+ * main() builds them (period_search_BOINC.cpp:343-866, trifac.c, areanorm.c,
+ * sphfunc.c, ellfit.c).  ps_input_view fills *view with pointers into the
+ * object (valid until ps_input_free) plus conw (header value) and n_max
+ * (number of trial frequencies). */
+typedef struct ps_input ps_input;
+int ps_input_read(const char* period_search_in, ps_input** out, char* err, int errlen);
+int ps_input_view(const ps_input* in, ps_problem* view, double* conw, int* n_max);
+void ps_input_free(ps_input* in);
+
+/* Best pole of one frequency (start_CUDA.cu:1200-1210): the earliest
+ * reported pole whose dev is not NaN and not larger than the incumbent's
+ * (a NaN incumbent is replaced).  r points to PS_N_POLES results; returns
+ * the pole index 0..9 or -1 if none is reported. */
+int ps_select_best_pole(const ps_pair_result* r);
+
+/* Library / context */
+int ps_open(int device, ps_ctx** out);
+const char* ps_last_error(const ps_ctx* ctx);
+int ps_close(ps_ctx* ctx);
+
+/* CUDAPrepare + the table uploads of CUDAPrecalc/CUDAStart: copies the
+ * problem (deep copy) to the device.  Replaces any previous problem. */
+int ps_prepare(ps_ctx* ctx, const ps_problem* prob);
+
+/* Parse period_search_in and build the geometry tables exactly as the
+ * reference's main() does (period_search_BOINC.cpp:343-866), then
+ * ps_prepare() the result. */
+int ps_prepare_file(ps_ctx* ctx, const char* period_search_in);
+
+/* Query derived sizes after ps_prepare*: ints[0..7] = ndata, n_curves, nfac,
+ * ncoef, ma, mfit, n_max, n_iter_max; dbls[0..3] = f_start, f_step, escl,
+ * conw (conw only after ps_prepare_file, else 0). */
+int ps_info(const ps_ctx* ctx, int* ints, double* dbls);
+
+/* One round of the convexity-weight search (CUDAPrecalc): fits the first 10
+ * trial frequencies x 10 poles with the three convexity rows weighted by
+ * *conw_r, and sets *new_conw = 1 if the mean dark-facet percentage of the
+ * best poles is below 1, else doubles *conw_r. */
+int ps_precalc(ps_ctx* ctx, double* conw_r, int* new_conw);
+
+/* The whole convexity-weight loop of main() (period_search_BOINC.cpp:
+ * 647-771): starts from conw/escl^2 (conw from the input file).  Needs
+ * ps_prepare_file.  Writes the final conwR. */
+int ps_conw_search(ps_ctx* ctx, double* conw_r);
+
+/* CUDAStart: scans frequencies n_start .. n_start+n_count-1 (1-based) with
+ * the final conw_r; writes one line per frequency into out[n_count].  The
+ * first line of the whole scan (n == 1) carries conw_r*escl^2 in the dark
+ * column, as the reference does. */
+int ps_start(ps_ctx* ctx, int n_start, int n_count, double conw_r, ps_line* out);
+
+/* Fits an explicit list of (frequency index n, pole m), both 1-based.
+ * conw_r weights the convexity rows; conw_r_dev is the value the dev
+ * computation subtracts with (conw_r for the scan, 0 for the conw search,
+ * start_CUDA.cu:476).  Results into out[npairs]. */
+int ps_fit_pairs(ps_ctx* ctx, int npairs, const int* n_list, const int* m_list,
+                 double conw_r, double conw_r_dev, ps_pair_result* out);
+
+/* Statistics of the last ps_fit_pairs / ps_start / ps_precalc call:
+ * device time of the fit kernel in milliseconds (HIP events on the library's
+ * stream), fit-kernel launches, pairs fitted, and the summed visible
+ * (point, facet) pairs, LM iterations and normal-equation evaluations of
+ * those pairs.  Any pointer may be NULL. */
+int ps_last_stats(const ps_ctx* ctx, double* kernel_ms, int* launches, long long* pairs,
+                  long long* vis_pairs, long long* iterations, long long* evaluations);
+
+/* CUDAUnprepare: frees the device copy of the problem. */
+int ps_unprepare(ps_ctx* ctx);
+
+/* ---- kernel-level entry points used by the parity tests ---- */
+
+/* One normal-equation evaluation (the reference's mrqcof, Start.cu:
+ * 4464-4498 + 2262-2828 + 4791-4807) at parameters a[ma] (a[i] = parameter
+ * i+1): alpha[mfit*mfit] (symmetric), beta[mfit], *chisq, area[nfac] (facet
+ * areas of a), jrows[(ndata+3)*(ma+1)] raw Jacobian rows (column 0 = model
+ * brightness, column c = d/da_c; column 1 is not computed).  Any output
+ * pointer may be NULL. */
+int ps_eval(ps_ctx* ctx, const double* a, double conw_r2_rows, double* alpha,
+            double* beta, double* chisq, double* area, double* jrows);
+
+/* The damped Gauss-Jordan solve of mrqmin step 1 (Start.cu:2838-3021) on
+ * the device: x = (alpha with diagonal*(1+lambda))^-1 beta.  Returns the
+ * reference error code (0, 1, 2) in *err. */
+int ps_gaussj(ps_ctx* ctx, int mfit, const double* alpha, const double* beta,
+              double lambda, double* x, int* err);
+
+/* Transcendental functions of the kernels evaluated on the device:
+ * out[3*i .. 3*i+2] = exp(x[i]), exp2(x[i]), sin(x[i]); out2[i] = cos(x[i]). */
+int ps_math_probe(ps_ctx* ctx, int n, const double* x, double* out, double* out2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

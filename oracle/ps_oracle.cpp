@@ -38,9 +38,8 @@
  *     ascending order (invisible facets add an exact zero);
  *   - column sums / curve brightness sums: sequential over points;
  *   - J^T W J, J^T W r and chi^2: one augmented product [J'|r]^T W [J'|r];
- *     points are cut into tiles of `tile` points per curve, tile g goes to
- *     wave g % n_waves, each wave chains its points in order, and the wave
- *     partials are added left to right.
+ *     every element is one fused multiply-add chain over all points in
+ *     order (data curves in file order, then the three convexity rows).
  * Transcendentals (exp, exp2, sin/cos of the LM parameters) use this file's
  * own Cody-Waite + Taylor implementations, which the kernel restates
  * identically; host-side setup uses the C library as the reference does.
@@ -819,28 +818,24 @@ EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order
                double* alpha, double* beta, double* chisq, double* area, double* jrows_out) {
   EvalOut out;
   const int ma = P.ma, mf = P.mfit, K = mf + 1; /* augmented width */
-  const int NW = ord.n_waves > 0 ? ord.n_waves : 1, TL = ord.tile > 0 ? ord.tile : 16;
+  const int TL = ord.tile > 0 ? ord.tile : 16; /* staging only: order-neutral */
   curv(P, a, area);
   PairInv v = pair_invariants(P, a);
   /* compact column map: compact q (0..mf-1) -> parameter index */
   std::vector<int> cmap;
   for (int c = 1; c <= ma; c++)
     if (P.ia[c]) cmap.push_back(c);
-  std::vector<double> acc((size_t)NW * K * K, 0.0); /* per-wave lower triangle incl. residual row */
+  std::vector<double> acc((size_t)K * K, 0.0); /* lower triangle incl. residual row */
   std::vector<double> rows((size_t)(P.max_lp > 3 ? P.max_lp : 3) * (ma + 1));
   std::vector<double> X((size_t)TL * K), Xw((size_t)TL * K);
-  long long gtile = 0;
   int pt0 = 0;
   auto syrk_tile = [&](int np, const double* Xt, const double* s2w) {
-    double* A = &acc[(size_t)(gtile % NW) * K * K];
-    for (int p = 0; p < TL; p++) {
-      if (p >= np) continue; /* zero-padded points add exact zeros */
+    double* A = acc.data();
+    for (int p = 0; p < np; p++)
       for (int i = 0; i < K; i++) {
         const double xi = Xt[p * K + i];
         for (int j = 0; j <= i; j++) A[i * K + j] = std::fma(xi, s2w[p] * Xt[p * K + j], A[i * K + j]);
       }
-    }
-    gtile++;
   };
   std::vector<double> s2w(TL);
   for (int ci = 0; ci < P.l_curves; ci++) {
@@ -905,11 +900,7 @@ EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order
     }
     syrk_tile(3, X.data(), s2w.data());
   }
-  /* combine wave partials left to right */
-  std::vector<double> tot(acc.begin(), acc.begin() + (size_t)K * K);
-  for (int w = 1; w < NW; w++)
-    for (int i = 0; i < K; i++)
-      for (int j = 0; j <= i; j++) tot[i * K + j] = tot[i * K + j] + acc[(size_t)w * K * K + i * K + j];
+  const std::vector<double>& tot = acc;
   for (int i = 0; i < mf; i++)
     for (int j = 0; j <= i; j++) {
       alpha[i * mf + j] = tot[i * K + j];
@@ -1221,6 +1212,7 @@ int pso_table(const pso_problem* h, const char* name, double* out, int cap) {
   else if (n == "l_points") { for (int x : P.l_points) v.push_back(x); }
   else if (n == "in_rel") { for (int x : P.in_rel) v.push_back(x); }
   else if (n == "pleg") v = P.pleg;
+  else if (n == "par") v = {P.par[1], P.par[2], P.par[3], P.lcl, 1.0};
   else return -1;
   (void)F; (void)N;
   if (out) {

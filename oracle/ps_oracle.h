@@ -16,9 +16,9 @@ extern "C" {
 /* Opaque parsed + set-up problem (input file, geometry tables). */
 typedef struct pso_problem pso_problem;
 
-/* Summation-order parameters the oracle mirrors so that its results can be
- * compared bit-for-bit with the GPU kernel: n_waves = wavefronts that split a
- * pair's point tiles round-robin; tile = points per tile (16). */
+/* Work-decomposition parameters of the GPU kernel (wavefronts per pair,
+ * points per tile).  The oracle's summation order does not depend on them;
+ * they are kept so callers can state the configuration they compare. */
 typedef struct {
   int n_waves;
   int tile;
@@ -53,7 +53,8 @@ void pso_info(const pso_problem* p, int* ints, double* dbls);
  *  "normal" [3*F], "d_area" [F], "d_sphere" [C*F] (k=1..C rows),
  *  "cg_first" [C], "tim" [N], "ee" [3*N], "ee0" [3*N], "alpha" [N]
  *  (solar phase per point), "brightness" [N], "sig" [N], "sigr2" [N],
- *  "weight" [N], "ia" [ma] (as doubles), "l_points" [L], "in_rel" [L].
+ *  "weight" [N], "ia" [ma] (as doubles), "l_points" [L], "in_rel" [L],
+ *  "pleg" [(mmax+1)(lmax+1)F], "par" [5] = phase A, D, k, log(cl), 1.
  * Returns number of elements, or -1. Pass out=NULL to query the size. */
 int pso_table(const pso_problem* p, const char* name, double* out, int cap);
 

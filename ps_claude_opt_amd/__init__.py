@@ -1,0 +1,188 @@
+"""MI355X period scan of PeriodSearch (convex lightcurve inversion).
+
+Python mirror of the reference's GPU layer interface (start_CUDA.h:1-18):
+``Scan.prepare`` / ``Scan.precalc`` / ``Scan.start`` / ``Scan.close`` map to
+CUDAPrepare / CUDAPrecalc / CUDAStart / CUDAUnprepare, on top of the C ABI
+(include/ps_mi355x.h) and the hand-written gfx950 kernels.  Errors are raised
+as ``PsError`` carrying the library's status code and message (the reference
+exits the process instead).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+
+__all__ = ["PsError", "Scan", "Line", "format_line", "write_output"]
+
+
+class PsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ps_mi355x error {code}: {msg}")
+        self.code = code
+
+
+@dataclass
+class Line:
+    period_h: float
+    rms: float
+    chi2: float
+    dark_or_conw: float
+    lambda_deg: float
+    beta_deg: float
+
+
+def format_line(L: Line) -> str:
+    """period_search_out line (start_CUDA.cu:1222-1226)."""
+    return "%.8f  %.6f  %.6f %4.1f %4.0f %4.0f\n" % (
+        L.period_h, L.rms, L.chi2, L.dark_or_conw, _c_round(L.lambda_deg), _c_round(L.beta_deg))
+
+
+def _c_round(x: float) -> float:
+    # C round(): halves away from zero
+    return float(np.sign(x) * np.floor(abs(x) + 0.5))
+
+
+def write_output(path: str, lines: Sequence[Line]) -> None:
+    with open(path, "w") as f:
+        for L in lines:
+            if not np.isnan(L.period_h):
+                f.write(format_line(L))
+
+
+class Scan:
+    """One device context holding one prepared problem."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _abi.load()
+        h = C.c_void_p()
+        rc = self._lib.ps_open(device, C.byref(h))
+        if rc != _abi.PS_OK:
+            raise PsError(rc, f"cannot open HIP device {device}")
+        self._h = h
+        self.info = None
+
+    # -- helpers -----------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != _abi.PS_OK:
+            raise PsError(rc, self._lib.ps_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.ps_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- reference interface ------------------------------------------------
+    def prepare_file(self, path: str):
+        """Parse period_search_in and upload the problem (CUDAPrepare)."""
+        self._check(self._lib.ps_prepare_file(self._h, path.encode()))
+        self._load_info()
+        return self
+
+    def _load_info(self):
+        ints = np.zeros(8, np.int32)
+        dbls = np.zeros(4, np.float64)
+        self._check(self._lib.ps_info(self._h, _abi.iptr(ints), _abi.dptr(dbls)))
+        keys = ("ndata", "n_curves", "nfac", "ncoef", "ma", "mfit", "n_max", "n_iter_max")
+        self.info = {k: int(v) for k, v in zip(keys, ints)}
+        self.info.update(f_start=float(dbls[0]), f_step=float(dbls[1]), escl=float(dbls[2]), conw=float(dbls[3]))
+
+    def precalc(self, conw_r: float):
+        """One convexity-weight round (CUDAPrecalc): returns (conw_r, new_conw)."""
+        cw = C.c_double(conw_r)
+        nc = C.c_int(0)
+        self._check(self._lib.ps_precalc(self._h, C.byref(cw), C.byref(nc)))
+        return cw.value, nc.value
+
+    def conw_search(self) -> float:
+        cw = C.c_double(0.0)
+        self._check(self._lib.ps_conw_search(self._h, C.byref(cw)))
+        return cw.value
+
+    def start(self, conw_r: float, n_start: int = 1, n_count: Optional[int] = None) -> List[Line]:
+        """Scan n_start..n_start+n_count-1 (CUDAStart); one Line per frequency."""
+        if n_count is None:
+            n_count = self.info["n_max"] - n_start + 1
+        arr = (_abi.PsLine * max(n_count, 1))()
+        self._check(self._lib.ps_start(self._h, n_start, n_count, conw_r, arr))
+        return [Line(a.period_h, a.rms, a.chi2, a.dark_or_conw, a.lambda_deg, a.beta_deg)
+                for a in arr[:n_count]]
+
+    def fit_pairs(self, n_list, m_list, conw_r: float, conw_r_dev: Optional[float] = None):
+        """Raw per-pair results as a structured numpy array."""
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        out = (_abi.PsPairResult * max(len(n_arr), 1))()
+        dev = conw_r if conw_r_dev is None else conw_r_dev
+        self._check(self._lib.ps_fit_pairs(self._h, len(n_arr), _abi.iptr(n_arr), _abi.iptr(m_arr),
+                                           conw_r, dev, out))
+        names = ("reported", "niter", "dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "n_vis")
+        res = np.zeros(len(n_arr), dtype=[(n, "f8") for n in names])
+        for i in range(len(n_arr)):
+            for n in names:
+                res[i][n] = getattr(out[i], n)
+        return res
+
+    def last_stats(self) -> dict:
+        """Device time and work counters of the last fit call."""
+        ms = C.c_double(0.0)
+        launches = C.c_int(0)
+        v = [C.c_longlong(0) for _ in range(4)]
+        self._check(self._lib.ps_last_stats(self._h, C.byref(ms), C.byref(launches), *[C.byref(x) for x in v]))
+        return dict(kernel_ms=ms.value, launches=launches.value, pairs=v[0].value, vis_pairs=v[1].value,
+                    iterations=v[2].value, evaluations=v[3].value)
+
+    def unprepare(self):
+        self._check(self._lib.ps_unprepare(self._h))
+
+    # -- kernel-level entry points (parity tests) ---------------------------
+    def eval(self, a: np.ndarray, conw_r2_rows: float, jrows: bool = False):
+        info = self.info
+        mf, F = info["mfit"], info["nfac"]
+        a = np.ascontiguousarray(a, np.float64)
+        alpha = np.zeros((mf, mf))
+        beta = np.zeros(mf)
+        chisq = np.zeros(1)
+        area = np.zeros(F)
+        jr = np.zeros((info["ndata"] + 3, info["ma"] + 1)) if jrows else None
+        self._check(self._lib.ps_eval(self._h, _abi.dptr(a), conw_r2_rows, _abi.dptr(alpha), _abi.dptr(beta),
+                                      _abi.dptr(chisq), _abi.dptr(area),
+                                      _abi.dptr(jr) if jrows else None))
+        return alpha, beta, float(chisq[0]), area, jr
+
+    def gaussj(self, alpha: np.ndarray, beta: np.ndarray, lam: float):
+        mf = len(beta)
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        beta = np.ascontiguousarray(beta, np.float64)
+        x = np.zeros(mf)
+        err = np.zeros(1, np.int32)
+        self._check(self._lib.ps_gaussj(self._h, mf, _abi.dptr(alpha), _abi.dptr(beta), lam, _abi.dptr(x),
+                                        _abi.iptr(err)))
+        return x, int(err[0])
+
+    def math_probe(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(3 * len(x))
+        out2 = np.zeros(len(x))
+        self._check(self._lib.ps_math_probe(self._h, len(x), _abi.dptr(x), _abi.dptr(out), _abi.dptr(out2)))
+        o = out.reshape(-1, 3)
+        return o[:, 0], o[:, 1], o[:, 2], out2
+
+
+def run_file(path_in: str, path_out: str, device: int = 0) -> float:
+    """The whole application on one input file; returns the final conwR."""
+    with Scan(device) as s:
+        s.prepare_file(path_in)
+        cw = s.conw_search()
+        write_output(path_out, s.start(cw))
+        return cw

@@ -1,0 +1,908 @@
+// MI355X (gfx950) kernels of the period scan: one persistent workgroup per
+// (trial frequency, initial pole) pair runs that pair's whole
+// Levenberg-Marquardt fit (the reference spreads one LM iteration over
+// ~13 + 4 * curves kernel launches, start_CUDA.cu:1017-1175).
+//
+// Per mrqcof evaluation (reference Start.cu:2262-2828):
+//   curv          facet areas Area_f = Darea_f exp(sum P_l^m (c cos + s sin))
+//   phase 1       per 16-point tile and 4-facet step, lane l owns point
+//                 (l & 15) and facet (4k + (l >> 4)); it evaluates the
+//                 scattering law and the derivative sums on the VALU and
+//                 feeds the facet weight w = Area * S as the A operand of
+//                 v_mfma_f64_16x16x4f64 against the shared basis table
+//                 (B operand), which yields the shape Jacobian of 16 points
+//                 x 16 coefficients per block.  Raw Jacobian rows go to a
+//                 per-workgroup scratch (L2-resident).
+//   curve sums    column sums d and brightness sum L of the curve
+//   phase 2       relative renormalisation J' = c (J - y d / L), residual
+//                 r; the augmented product [J'|r]^T W [J'|r] is one FP64
+//                 MFMA SYRK whose lower blocks stay in registers: it gives
+//                 alpha, beta and chi^2 at once.
+// mrqmin: full-pivot Gauss-Jordan in LDS, accept / reject, Iter2, FinishPole
+// (Start.cu:2838-3021, 300-392, 5129-5248, 5056-5125).
+//
+// Operation order is fixed (see the CPU oracle): facet partial sums by
+// facet mod 4 combined (P0+P1)+(P2+P3); the MFMA gather is a fused
+// multiply-add chain over facets in ascending order; every element of the
+// augmented normal matrix is one fused multiply-add chain over all points in
+// order (the 16x16 block pairs are split across the waves, every wave walks
+// every tile).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ps_math.h"
+#include "ps_kernels.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int NW = PS_FIT_WAVES;  // waves per workgroup
+constexpr int WG = 64 * NW;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTiny = 1e-8;  // constants.h:19
+
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// 1/x: hardware reciprocal refined by two Newton steps (agrees with the
+// correctly rounded quotient; PS_IEEE_DIV selects the IEEE division).
+__device__ __forceinline__ double recip(double x) {
+#ifdef PS_IEEE_DIV
+  return 1.0 / x;
+#else
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#endif
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double xsum16_32(double v) {
+  v = v + __shfl_xor(v, 16);
+  v = v + __shfl_xor(v, 32);
+  return v;
+}
+
+struct Inv {
+  double omega, A, invD, kph, invD2, cl, cls, sb, cb, sl, clam;
+};
+
+// Pair invariants of one parameter vector a[] (a[i] = parameter i+1).
+__device__ Inv make_inv(const PsDevProblem& P, const double* a) {
+  const int C = P.ncoef;
+  Inv v;
+  v.omega = a[C + 2];
+  v.A = a[C + 3];
+  v.invD = 1.0 / a[C + 4];
+  v.kph = a[C + 5];
+  v.invD2 = v.invD * v.invD;
+  v.cl = psm::exp(a[C + 6]);
+  v.cls = a[C + 7];
+  psm::sincos(a[C], &v.sb, &v.cb);
+  psm::sincos(a[C + 1], &v.sl, &v.clam);
+  return v;
+}
+
+// Body-frame direction g = Rz(phi) P(beta') Rz(lambda) e and its
+// derivatives with respect to beta', lambda and omega.
+__device__ __forceinline__ void body_vectors(const Inv& v, double cf, double sf, double t, double ex, double ey,
+                                             double ez, double* g, double* db, double* dl, double* dw) {
+  const double ux = fma(v.sl, ey, v.clam * ex);
+  const double uy = fma(-v.sl, ex, v.clam * ey);
+  const double uz = ez;
+  const double vx = fma(-v.sb, uz, v.cb * ux);
+  const double vy = uy;
+  const double vz = fma(v.sb, ux, v.cb * uz);
+  g[0] = fma(sf, vy, cf * vx);
+  g[1] = fma(-sf, vx, cf * vy);
+  g[2] = vz;
+  db[0] = -(cf * vz);
+  db[1] = sf * vz;
+  db[2] = vx;
+  const double w = v.cb * uy;
+  dl[0] = fma(cf, w, -(sf * ux));
+  dl[1] = fma(-sf, w, -(cf * ux));
+  dl[2] = v.sb * uy;
+  dw[0] = t * g[1];
+  dw[1] = -(t * g[0]);
+  dw[2] = 0.0;
+}
+
+struct Geo {
+  double g[3], g0[3], db[3], db0[3], dl[3], dl0[3], dw[3], dw0[3];
+  double scale, ff, d2, alph;
+};
+
+__device__ __forceinline__ void point_geometry(const PsDevProblem& P, const Inv& v, int pt, Geo& G) {
+  const double t = P.tim[pt];
+  double f = fma(v.omega, t, P.phi0);
+  f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
+  double sf, cf;
+  psm::sincos(f, &sf, &cf);
+  const double alph = P.alph[pt];
+  G.alph = alph;
+  G.ff = psm::exp2(-1.44269504088896 * (alph * v.invD));
+  G.scale = fma(v.kph, alph, fma(v.A, G.ff, 1.0));
+  G.d2 = ((v.A * G.ff) * alph) * v.invD2;
+  body_vectors(v, cf, sf, t, P.ex[pt], P.ey[pt], P.ez[pt], G.g, G.db, G.dl, G.dw);
+  body_vectors(v, cf, sf, t, P.sx[pt], P.sy[pt], P.sz[pt], G.g0, G.db0, G.dl0, G.dw0);
+}
+
+// LDS layout of one workgroup (dynamic shared memory).
+struct Lds {
+  double* work;    // [K*K] normal matrix target / Gauss-Jordan matrix
+  double* area;    // [F]
+  double* cgg;     // [ma]
+  double* atry;    // [ma]
+  double* dvec;    // [qs] column sums of the current curve
+  double* rowbuf;  // [NW][16][16] per-point non-shape values
+  double* x;       // [K] right-hand side / solution
+  double* dum;     // [K] elimination multipliers
+  double* red;     // [2*NW] reduction scratch
+  int* ipiv;       // [K]
+  int* ired;       // [2*NW]
+};
+
+__device__ Lds carve_lds(double* base, const PsDevProblem& P) {
+  Lds L;
+  const int K = P.mfit + 1;
+  double* p = base;
+  L.work = p;
+  p += K * K;
+  L.area = p;
+  p += P.nfac;
+  L.cgg = p;
+  p += P.ma;
+  L.atry = p;
+  p += P.ma;
+  L.dvec = p;
+  p += P.qs;
+  L.rowbuf = p;
+  p += NW * 256;
+  L.x = p;
+  p += K;
+  L.dum = p;
+  p += K;
+  L.red = p;
+  p += 2 * NW;
+  L.ipiv = reinterpret_cast<int*>(p);
+  p += (K + 1) / 2 + 1;
+  L.ired = reinterpret_cast<int*>(p);
+  return L;
+}
+
+// Facet areas of parameter vector a (Start.cu:421-528).
+__device__ void curv_block(const PsDevProblem& P, const double* a, double* area) {
+  const int F = P.nfac, Lm = P.lmax, M = P.mmax;
+  for (int f = threadIdx.x; f < F; f += WG) {
+    double g = 0.0;
+    int n = 0;
+    for (int l = 0; l <= Lm; ++l) {
+      const double fsum = a[n++];
+      g = fma(P.pleg[l * F + f], fsum, g);
+    }
+    for (int m = 1; m <= M; ++m) {
+      const double fcm = P.fc[m * F + f], fsm = P.fs[m * F + f];
+      for (int l = m; l <= Lm; ++l) {
+        double fsum = a[n] * fcm;
+        fsum = fma(a[n + 1], fsm, fsum);
+        n += 2;
+        g = fma(P.pleg[(m * (Lm + 1) + l) * F + f], fsum, g);
+      }
+    }
+    area[f] = P.darea[f] * psm::exp(g);
+  }
+}
+
+// Phase 1 for one 16-point tile of one curve, executed by one wave.
+template <int NB, int NBS>
+__device__ unsigned phase1_tile(const PsDevProblem& P, const Inv& v, const double* area, int p0, int lp, int t,
+                                double* jbuf, double* ybuf, double* rowbuf) {
+  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
+  const int F = P.nfac, QS = P.qs, C = P.ncoef;
+  const int pin = t * 16 + ps;
+  const bool valid = pin < lp;
+  const int pt = p0 + (valid ? pin : 0);
+  // point geometry: only the body-frame directions stay live in the loop
+  double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0}, scr[3];
+  double sf = 0.0, cf = 0.0, scale = 0.0, ff = 0.0, d2 = 0.0, alph = 0.0;
+  if (valid) {
+    const double tt = P.tim[pt];
+    double f = fma(v.omega, tt, P.phi0);
+    f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
+    psm::sincos(f, &sf, &cf);
+    alph = P.alph[pt];
+    ff = psm::exp2(-1.44269504088896 * (alph * v.invD));
+    scale = fma(v.kph, alph, fma(v.A, ff, 1.0));
+    d2 = ((v.A * ff) * alph) * v.invD2;
+    body_vectors(v, cf, sf, tt, P.ex[pt], P.ey[pt], P.ez[pt], g, scr, scr, scr);
+    body_vectors(v, cf, sf, tt, P.sx[pt], P.sy[pt], P.sz[pt], g0, scr, scr, scr);
+  }
+  double br = 0.0, t4 = 0.0, t5 = 0.0;
+  double U0 = 0.0, U1 = 0.0, U2 = 0.0, V0 = 0.0, V1 = 0.0, V2 = 0.0;
+  v4d gat[NBS];
+#pragma unroll
+  for (int b = 0; b < NBS; ++b) gat[b] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double cl = v.cl, cls = v.cls;
+  const double* __restrict__ nxp = P.nx;
+  const double* __restrict__ nyp = P.ny;
+  const double* __restrict__ nzp = P.nz;
+  const double* __restrict__ yg = P.yg;
+  unsigned nvis = 0;
+  for (int k = 0; k < F / 4; ++k) {
+    const int f = 4 * k + sl;
+    const double n0 = nxp[f], n1 = nyp[f], n2 = nzp[f];
+    const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
+    const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
+    const bool vis = (mu > kTiny) && (mu0 > kTiny);
+    const unsigned long long bal = __ballot(vis);
+    if (bal == 0) continue;  // whole step invisible: exact zeros
+    nvis += (unsigned)__popcll(bal);
+    double w = 0.0;
+    if (vis) {
+      const double ar = area[f];
+      const double rd = recip(mu + mu0);
+      const double mm = mu * mu0;
+      const double sc = mm * fma(cls, rd, cl);
+      w = ar * sc;
+      const double q0 = mu0 * rd, q1 = mu * rd;
+      const double dsmu = fma(cls, q0 * q0, cl * mu0);
+      const double dsmu0 = fma(cls, q1 * q1, cl * mu);
+      const double ad = ar * dsmu, ad0 = ar * dsmu0;
+      U0 = fma(ad, n0, U0);
+      U1 = fma(ad, n1, U1);
+      U2 = fma(ad, n2, U2);
+      V0 = fma(ad0, n0, V0);
+      V1 = fma(ad0, n1, V1);
+      V2 = fma(ad0, n2, V2);
+      br = br + w;
+      t4 = fma(ar, mm, t4);
+      t5 = fma(ar * mm, rd, t5);
+    }
+    const double* yrow = yg + (size_t)f * QS + ps;
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, yrow[16 * b], gat[b]);
+  }
+  br = xsum16_32(br);
+  t4 = xsum16_32(t4);
+  t5 = xsum16_32(t5);
+  U0 = xsum16_32(U0);
+  U1 = xsum16_32(U1);
+  U2 = xsum16_32(U2);
+  V0 = xsum16_32(V0);
+  V1 = xsum16_32(V1);
+  V2 = xsum16_32(V2);
+  if (sl == 0) {
+    double db[3], dl[3], dw[3], db0[3], dl0[3], dw0[3], gg[3];
+    const double tt = valid ? P.tim[pt] : 0.0;
+    body_vectors(v, cf, sf, tt, valid ? P.ex[pt] : 0.0, valid ? P.ey[pt] : 0.0, valid ? P.ez[pt] : 0.0, gg, db,
+                 dl, dw);
+    body_vectors(v, cf, sf, tt, valid ? P.sx[pt] : 0.0, valid ? P.sy[pt] : 0.0, valid ? P.sz[pt] : 0.0, gg,
+                 db0, dl0, dw0);
+    auto tsum = [&](const double* dv, const double* dv0) {
+      double r = dv[0] * U0;
+      r = fma(dv[1], U1, r);
+      r = fma(dv[2], U2, r);
+      r = fma(dv0[0], V0, r);
+      r = fma(dv0[1], V1, r);
+      r = fma(dv0[2], V2, r);
+      return r;
+    };
+    const double T1 = tsum(db, db0), T2 = tsum(dl, dl0), T3 = tsum(dw, dw0);
+    const double sc = scale;
+    double* rb = rowbuf + ps * 16;
+    rb[0] = sc * T1;
+    rb[1] = sc * T2;
+    rb[2] = sc * T3;
+    rb[3] = br * ff;
+    rb[4] = br * d2;
+    rb[5] = br * alph;
+    rb[6] = (sc * t4) * cl;
+    rb[7] = sc * t5;
+    rb[8] = br * sc;
+    rb[9] = sc;
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ptl = 4 * r + sl;
+    const int prow = t * 16 + ptl;
+    if (prow < lp) {
+      const double* rb = rowbuf + ptl * 16;
+      double* jr = jbuf + (size_t)prow * QS;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int q = 16 * b + ps;
+        double val = 0.0;
+        if (q <= C - 2) {
+          if (b < NBS) val = rb[9] * gat[b < NBS ? b : 0][r];
+        } else if (q <= C + 6) {
+          val = rb[q - (C - 1)];
+        }
+        jr[q] = val;
+      }
+      if (ps == 0) ybuf[prow] = rb[8];
+    }
+  }
+  wave_sync();
+  return nvis;
+}
+
+// Phase 2 for one tile: X = [J' | r], W = s2w; the augmented normal matrix
+// gains X^T (W X).  Every wave walks every tile and owns the 16x16 block
+// pairs (ib >= jb) whose enumeration index is = wave (mod NW).
+// `conv` selects the three convexity rows (no renormalisation).
+template <int NB>
+__device__ void phase2_tile(const PsDevProblem& P, int p0, int lp, int t, const double* jbuf, const double* ybuf,
+                            const double* dvec, double rave, bool conv, double conv_w, v4d* acc) {
+  constexpr int NP = NB * (NB + 1) / 2;
+  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
+  const int mf = P.mfit, QS = P.qs, C = P.ncoef;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int pin = t * 16 + 4 * r + sl;
+    const bool valid = pin < lp;
+    double coef = 0.0, coef1 = 0.0, res = 0.0, s2 = 0.0;
+    if (valid) {
+      const double y = ybuf[pin];
+      if (conv) {
+        res = 0.0 - y;
+        s2 = conv_w * 1.0;
+      } else {
+        const int p = p0 + pin;
+        coef = (P.sig[p] * (double)lp) * rave;
+        coef1 = y * rave;
+        const double ym = coef * y;
+        res = P.bright[p] - ym;
+        s2 = P.s2w[p];
+      }
+    }
+    const double* jr = jbuf + (size_t)(valid ? pin : 0) * QS;
+    double X[NB], XW[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int q = 16 * b + ps;
+      double xv = 0.0;
+      if (valid) {
+        if (q < mf) {
+          if (conv)
+            xv = (q + 2 <= C) ? jr[q] : 0.0;
+          else
+            xv = coef * fma(-coef1, dvec[q], jr[q]);
+        } else if (q == mf) {
+          xv = res;
+        }
+      }
+      X[b] = xv;
+      XW[b] = s2 * xv;
+    }
+    int idx = 0;
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int jb = 0; jb <= ib; ++jb) {
+        if (idx % NW == w) acc[idx / NW] = mfma(X[ib], XW[jb], acc[idx / NW]);
+        ++idx;
+      }
+    (void)NP;
+  }
+}
+
+// One mrqcof evaluation at a[] (LDS) into L.work (K*K, symmetric alpha,
+// beta in row mfit, chi^2 at [mfit][mfit]); L.area = curv(a).
+template <int NB, int NBS>
+__device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, double* jbuf,
+                         double* ybuf, unsigned long long* vis_count) {
+  constexpr int NP = NB * (NB + 1) / 2;
+  constexpr int NPW = (NP + NW - 1) / NW;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, ps = lane & 15, sl = lane >> 4;
+  const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac;
+  curv_block(P, a, L.area);
+  const Inv v = make_inv(P, a);
+  __syncthreads();
+  v4d acc[NPW];
+#pragma unroll
+  for (int i = 0; i < NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+  double* rowbuf = L.rowbuf + w * 256;
+  unsigned long long nvis = 0;
+  for (int c = 0; c < P.ncurves; ++c) {
+    const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
+    const int ntile = (lp + 15) / 16;
+    for (int t = w; t < ntile; t += NW) nvis += phase1_tile<NB, NBS>(P, v, L.area, p0, lp, t, jbuf, ybuf, rowbuf);
+    __syncthreads();
+    // curve sums, sequential over points (column q = parameter q+2)
+    for (int q = tid; q <= mf; q += WG) {
+      double s = 0.0;
+      if (q < mf) {
+        for (int j = 0; j < lp; ++j) s = s + jbuf[(size_t)j * QS + q];
+      } else {
+        for (int j = 0; j < lp; ++j) s = s + ybuf[j];
+      }
+      L.dvec[q] = s;
+    }
+    __syncthreads();
+    const double rave = 1.0 / L.dvec[mf];
+    for (int t = 0; t < ntile; ++t) phase2_tile<NB>(P, p0, lp, t, jbuf, ybuf, L.dvec, rave, false, 0.0, acc);
+    __syncthreads();
+  }
+  if (vis_count && lane == 0) atomicAdd(vis_count, nvis);
+  // convexity pseudo-curve (Start.cu:2539-2612): y_j = sum Area n_j,
+  // J_j[c] = sum (Area n_j) Y_c
+  for (int idx = tid; idx < 3 * C; idx += WG) {
+    const int j = idx / C, c = idx - j * C;
+    const double* nj = (j == 0) ? P.nx : (j == 1) ? P.ny : P.nz;
+    double s = 0.0;
+    if (c == 0) {
+      for (int f = 0; f < F; ++f) s = s + L.area[f] * nj[f];
+      ybuf[j] = s;
+    } else {
+      for (int f = 0; f < F; ++f) s = fma(L.area[f] * nj[f], P.yg[(size_t)f * QS + (c - 1)], s);
+      jbuf[(size_t)j * QS + (c - 1)] = s;
+    }
+  }
+  __syncthreads();
+  phase2_tile<NB>(P, 0, 3, 0, jbuf, ybuf, L.dvec, 0.0, true, conw_r2, acc);
+  // each wave stores its block pairs (lower triangle)
+  {
+    int idx = 0;
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int jb = 0; jb <= ib; ++jb) {
+        if (idx % NW == w) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int ai = 16 * ib + 4 * r + sl, bj = 16 * jb + ps;
+            if (ai < K && bj <= ai) L.work[ai * K + bj] = acc[idx / NW][r];
+          }
+        }
+        ++idx;
+      }
+  }
+  __syncthreads();
+  for (int e = tid; e < mf * mf; e += WG) {
+    const int i = e / mf, j = e - i * mf;
+    if (j > i) L.work[i * K + j] = L.work[j * K + i];
+  }
+  __syncthreads();
+}
+
+// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) on L.work (stride K,
+// mf x mf) with right-hand side L.x.  NR gaussj pivot rule: the last
+// occurrence of the largest |entry| in row-major order.  Returns the
+// reference error code (0, 1 = ipiv > 1, 2 = zero pivot / nothing to pick).
+__device__ int gaussj_block(const Lds& L, int mf, int K) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  for (int j = tid; j < mf; j += WG) L.ipiv[j] = 0;
+  __syncthreads();
+  for (int i = 0; i < mf; ++i) {
+    // row scan: thread j scans row j
+    double big = 0.0;
+    int brow = -1, bcol = -1, e1 = 0;
+    for (int j = tid; j < mf; j += WG) {
+      if (L.ipiv[j] == 1) continue;
+      for (int k = 0; k < mf; ++k) {
+        const int ik = L.ipiv[k];
+        if (ik == 0) {
+          const double tv = fabs(L.work[j * K + k]);
+          if (tv >= big) {
+            big = tv;
+            brow = j;
+            bcol = k;
+          }
+        } else if (ik > 1) {
+          e1 = 1;
+        }
+      }
+    }
+    // block argmax: larger |value|, ties -> larger row
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(big, off);
+      const int orow = __shfl_xor(brow, off), ocol = __shfl_xor(bcol, off);
+      const bool take = orow >= 0 && (brow < 0 || ob > big || (ob == big && orow > brow));
+      if (take) {
+        big = ob;
+        brow = orow;
+        bcol = ocol;
+      }
+    }
+    const int e1w = __any(e1);
+    if (lane == 0) {
+      L.red[w] = big;
+      L.ired[w] = brow;
+      L.ired[NW + w] = bcol;
+      L.red[NW + w] = e1w;
+    }
+    __syncthreads();
+    int err = 0, irow = -1, icol = -1;
+    {
+      double b = 0.0;
+      for (int ww = 0; ww < NW; ++ww) {
+        if (L.red[NW + ww] != 0.0) err = 1;
+        const int orow = L.ired[ww];
+        const double ob = L.red[ww];
+        if (orow >= 0 && (irow < 0 || ob > b || (ob == b && orow > irow))) {
+          b = ob;
+          irow = orow;
+          icol = L.ired[NW + ww];
+        }
+      }
+    }
+    if (err) return 1;
+    if (icol < 0) return 2;
+    __syncthreads();
+    if (tid == 0) L.ipiv[icol] += 1;
+    if (irow != icol) {
+      for (int l = tid; l < mf; l += WG) {
+        const double tmp = L.work[irow * K + l];
+        L.work[irow * K + l] = L.work[icol * K + l];
+        L.work[icol * K + l] = tmp;
+      }
+      if (tid == 0) {
+        const double tmp = L.x[irow];
+        L.x[irow] = L.x[icol];
+        L.x[icol] = tmp;
+      }
+    }
+    __syncthreads();
+    const double piv = L.work[icol * K + icol];
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    __syncthreads();
+    if (tid == 0) {
+      L.work[icol * K + icol] = 1.0;
+      L.x[icol] *= pivinv;
+    }
+    __syncthreads();
+    for (int l = tid; l < mf; l += WG) L.work[icol * K + l] *= pivinv;
+    for (int l = tid; l < mf; l += WG) L.dum[l] = L.work[l * K + icol];
+    __syncthreads();
+    for (int e = tid; e < mf * mf; e += WG) {
+      const int ll = e / mf, c = e - ll * mf;
+      if (ll == icol) continue;
+      const double dm = L.dum[ll];
+      const double base = (c == icol) ? 0.0 : L.work[ll * K + c];
+      L.work[ll * K + c] = fma(-L.work[icol * K + c], dm, base);
+    }
+    for (int ll = tid; ll < mf; ll += WG)
+      if (ll != icol) L.x[ll] = fma(-L.x[icol], L.dum[ll], L.x[ll]);
+    __syncthreads();
+  }
+  return 0;
+}
+
+struct PairState {
+  double alamda, chisq, ochisq, iter_diff, dev_old, dev_new, chck;
+  unsigned long long nvis;  // visible (point, facet) pairs over all evaluations
+  int niter, flags;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Persistent LM fit kernel.
+// ---------------------------------------------------------------------------
+template <int NB, int NBS>
+__global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
+  extern __shared__ double lds_dyn[];
+  __shared__ PairState S;
+  __shared__ int item_s;
+  const PsDevProblem& P = A.P;
+  const Lds L = carve_lds(lds_dyn, P);
+  const int tid = threadIdx.x;
+  const int mf = P.mfit, K = mf + 1, C = P.ncoef, ma = P.ma, F = P.nfac;
+  double* jbuf = A.scratch + (size_t)blockIdx.x * A.scratch_stride;
+  double* ybuf = jbuf + (size_t)A.jrows * P.qs;
+  double* best = ybuf + A.jrows;  // [K*K] accepted alpha | beta | chi^2
+  for (;;) {
+    if (tid == 0) item_s = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int item = item_s;
+    __syncthreads();
+    if (item >= A.n_items) break;
+    const int n = A.item_n[item], m = A.item_m[item];  // 1-based
+    // pair initialisation (Start.cu:4346-4418)
+    if (tid == 0) {
+      const double period = 1.0 / (P.f_start - (double)(n - 1) * P.f_step);
+      const double d2r = kPi / 180.0;
+      L.cgg[C] = d2r * 90 - d2r * P.beta_pole[m - 1];
+      L.cgg[C + 1] = d2r * P.lambda_pole[m - 1];
+      L.cgg[C + 2] = (24.0 * 2.0 * kPi) / period;
+      L.cgg[C + 3] = P.par[0];
+      L.cgg[C + 4] = P.par[1];
+      L.cgg[C + 5] = P.par[2];
+      L.cgg[C + 6] = P.lcl;
+      L.cgg[C + 7] = 1;
+      S.alamda = -1;
+      S.niter = 0;
+      S.iter_diff = 1e40;
+      S.dev_old = 1e30;
+      S.dev_new = 0;
+      S.chck = 0;
+      S.chisq = 0;
+      S.ochisq = 0;
+      S.nvis = 0;
+    }
+    for (int i = tid; i < C; i += WG) L.cgg[i] = P.cg_first[i];
+    __syncthreads();
+    for (;;) {
+      // Iter1Begin (Start.cu:4421-4462)
+      const int niter = S.niter;
+      const bool is_niter = ((niter < P.n_iter_max) && (S.iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min);
+      if (!is_niter) break;
+      const bool is_alambda = S.alamda < 0;
+      __syncthreads();
+      if (is_alambda && tid == 0) S.alamda = P.alamda_start;
+      // pass 0 (first iteration only): normal equations at cgg;
+      // pass 1: damped solve (mrqmin step 1, Start.cu:2838-3021), then the
+      // normal equations at the trial point atry.
+#pragma unroll 1
+      for (int pass = is_alambda ? 0 : 1; pass < 2; ++pass) {
+        if (pass == 1) {
+          const double ccc = 1 + S.alamda;
+          for (int e = tid; e < mf * mf; e += WG) {
+            const int i = e / mf, j = e - i * mf;
+            double vv = best[i * K + j];
+            if (i == j) vv *= ccc;
+            L.work[i * K + j] = vv;
+          }
+          for (int j = tid; j < mf; j += WG) L.x[j] = best[mf * K + j];
+          __syncthreads();
+          const int err = gaussj_block(L, mf, K);
+          __syncthreads();
+          if (err == 0 || err == 2)
+            for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.x[q];
+          __syncthreads();
+        }
+        evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, A.conw_r2_rows, jbuf, ybuf, &S.nvis);
+        if (pass == 0) {
+          for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
+          if (tid == 0) S.ochisq = L.work[mf * K + mf];
+          for (int i = tid; i < ma; i += WG) L.atry[i] = L.cgg[i];
+          __syncthreads();
+        }
+      }
+      const double chisq_try = L.work[mf * K + mf];
+      // mrqmin step 2 (Start.cu:300-392)
+      const bool accept = chisq_try < S.ochisq;
+      if (accept) {
+        for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
+        for (int i = tid; i < ma; i += WG) L.cgg[i] = L.atry[i];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        if (accept) {
+          S.alamda = S.alamda * P.alamda_incrr;
+          S.chisq = chisq_try;
+        } else {
+          S.alamda = P.alamda_incr * S.alamda;
+          S.chisq = S.ochisq;
+        }
+        S.niter = S.niter + 1;
+      }
+      __syncthreads();
+      // Iter2 (Start.cu:5129-5248)
+      if (S.niter == 1 || S.chisq < S.ochisq) {
+        curv_block(P, L.cgg, L.area);
+        __syncthreads();
+        if (tid < 3) {
+          const double* nj = (tid == 0) ? P.nx : (tid == 1) ? P.ny : P.nz;
+          double s = 0.0;
+          for (int f = 0; f < F; ++f) s = fma(L.area[f], nj[f], s);
+          L.red[tid] = s;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          S.ochisq = S.chisq;
+          const double v0 = L.red[0], v1 = L.red[1], v2 = L.red[2];
+          const double aa = fma(v2, v2, fma(v1, v1, v0 * v0));
+          const double conwr2 = A.conw_r_dev * A.conw_r_dev;
+          const double rchisq = fma(-aa, conwr2, S.chisq);
+          S.dev_new = sqrt(rchisq / (double)P.ndata);
+          S.chck = sqrt(aa);
+          const double diff = S.dev_old - S.dev_new;
+          if (diff > 1e-10) {
+            S.iter_diff = diff;
+            S.dev_old = S.dev_new;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // FinishPole (Start.cu:5056-5125)
+    if (tid == 0) {
+      double* o = A.out + (size_t)item * PS_OUT_STRIDE;
+      double dev = 1e40, dark = 0, per = 0, la = 0, be = 0;
+      if (!(S.dev_new >= 1e40)) {
+        double tot = 0.0;
+        for (int f = 0; f < F; ++f) tot = tot + L.area[f];
+        tot = 1.0 / tot;
+        double dd = (S.chck * 100.0) * tot;
+        if (dd != dd) dd = 1.0;
+        dev = S.dev_new;
+        dark = dd;
+        per = (2 * kPi) / L.cgg[C + 2];
+        const double r2d = 180.0 / kPi;
+        const double lt = r2d * L.cgg[C + 1];
+        la = lt + (lt < 0 ? 360 : 0);
+        be = 90 - r2d * L.cgg[C];
+      }
+      o[0] = dev;
+      o[1] = dark;
+      o[2] = per;
+      o[3] = la;
+      o[4] = be;
+      o[5] = S.chisq;
+      o[6] = (double)S.niter;
+      o[7] = 1.0;
+      o[8] = (double)S.nvis;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single evaluation / Gauss-Jordan / math kernels for the parity tests.
+// ---------------------------------------------------------------------------
+template <int NB, int NBS>
+__global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* a, double* out_work,
+                                                     double* out_area, double* out_j) {
+  extern __shared__ double lds_dyn[];
+  const PsDevProblem& P = A.P;
+  const Lds L = carve_lds(lds_dyn, P);
+  const int tid = threadIdx.x, K = P.mfit + 1;
+  double* jbuf = A.scratch;
+  double* ybuf = jbuf + (size_t)A.jrows * P.qs;
+  for (int i = tid; i < P.ma; i += WG) L.cgg[i] = a[i];
+  __syncthreads();
+  if (out_j == nullptr) {
+    evaluate<NB, NBS>(P, L, L.cgg, A.conw_r2_rows, jbuf, ybuf, nullptr);
+  } else {
+    // raw Jacobian rows of every data point: phase 1 only, curve by curve
+    curv_block(P, L.cgg, L.area);
+    const Inv v = make_inv(P, L.cgg);
+    __syncthreads();
+    const int w = tid >> 6;
+    for (int c = 0; c < P.ncurves; ++c) {
+      const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
+      for (int t = w; t < (lp + 15) / 16; t += NW)
+        phase1_tile<NB, NBS>(P, v, L.area, p0, lp, t, jbuf, ybuf, L.rowbuf + w * 256);
+      __syncthreads();
+      for (int e = tid; e < lp * (P.ma + 1); e += WG) {
+        const int j = e / (P.ma + 1), col = e - j * (P.ma + 1);
+        double val;
+        if (col == 0)
+          val = ybuf[j];
+        else if (col == 1)
+          val = 0.0;
+        else
+          val = (col - 2 < P.qs) ? jbuf[(size_t)j * P.qs + (col - 2)] : 0.0;
+        out_j[(size_t)(p0 + j) * (P.ma + 1) + col] = val;
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < K * K; e += WG) out_work[e] = L.work[e];
+  for (int f = tid; f < P.nfac; f += WG) out_area[f] = L.area[f];
+}
+
+__global__ __launch_bounds__(WG) void ps_gaussj_kernel(int mf, const double* alpha, const double* beta, double lambda,
+                                                       double* x, int* err) {
+  extern __shared__ double lds_dyn[];
+  const int K = mf + 1;
+  Lds L;
+  L.work = lds_dyn;
+  L.x = L.work + K * K;
+  L.dum = L.x + K;
+  L.red = L.dum + K;
+  L.ipiv = reinterpret_cast<int*>(L.red + 2 * NW);
+  L.ired = L.ipiv + K + 1;
+  const int tid = threadIdx.x;
+  const double ccc = 1 + lambda;
+  for (int e = tid; e < mf * mf; e += WG) {
+    const int i = e / mf, j = e - i * mf;
+    double v = alpha[e];
+    if (i == j) v *= ccc;
+    L.work[i * K + j] = v;
+  }
+  for (int j = tid; j < mf; j += WG) L.x[j] = beta[j];
+  __syncthreads();
+  const int e = gaussj_block(L, mf, K);
+  __syncthreads();
+  for (int j = tid; j < mf; j += WG) x[j] = L.x[j];
+  if (tid == 0) *err = e;
+}
+
+__global__ void ps_math_kernel(int n, const double* x, double* out, double* out2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s, c;
+  psm::sincos(x[i], &s, &c);
+  out[3 * i + 0] = psm::exp(x[i]);
+  out[3 * i + 1] = psm::exp2(x[i]);
+  out[3 * i + 2] = s;
+  out2[i] = c;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+size_t ps_fit_lds_bytes(const PsDevProblem& P) {
+  const int K = P.mfit + 1;
+  size_t d = (size_t)K * K + P.nfac + 2 * P.ma + P.qs + NW * 256 + 2 * K + 2 * NW + (K + 1) / 2 + 1 + 2 * NW;
+  return d * sizeof(double) + 64;
+}
+
+size_t ps_fit_scratch_stride(const PsDevProblem& P, int jrows) {
+  const int K = P.mfit + 1;
+  size_t s = (size_t)jrows * P.qs + jrows + (size_t)K * K;
+  return (s + 31) & ~(size_t)31;
+}
+
+// (NB, NBS) instantiations: NB = 16-column blocks of a Jacobian row, NBS =
+// blocks holding shape columns (NBS in {NB-1, NB}).
+#define PS_FOR_EACH_SHAPE(X) X(1, 1) X(2, 1) X(2, 2) X(3, 2) X(3, 3) X(4, 3) X(4, 4) X(5, 4) X(5, 5) X(6, 5) X(6, 6)
+
+hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s) {
+  const size_t lds = ps_fit_lds_bytes(A.P);
+#define PS_L(NB_, NBS_)                                                                                   \
+  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                 \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_fit_kernel<NB_, NBS_>), dim3(grid), dim3(WG), lds, s, A);       \
+    return hipGetLastError();                                                                             \
+  }
+  PS_FOR_EACH_SHAPE(PS_L)
+#undef PS_L
+  return hipErrorInvalidValue;
+}
+
+hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
+                          double* jrows) {
+  const size_t lds = ps_fit_lds_bytes(A.P);
+#define PS_L(NB_, NBS_)                                                                                   \
+  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                 \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_>), dim3(1), dim3(WG), lds, s, A, a, work,  \
+                       area, jrows);                                                                      \
+    return hipGetLastError();                                                                             \
+  }
+  PS_FOR_EACH_SHAPE(PS_L)
+#undef PS_L
+  return hipErrorInvalidValue;
+}
+
+hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda, double* x, int* err,
+                            hipStream_t s) {
+  const int K = mf + 1;
+  const size_t lds = ((size_t)K * K + 2 * K + 2 * NW + K + 2 + 2 * NW) * sizeof(double);
+  hipLaunchKernelGGL(ps_gaussj_kernel, dim3(1), dim3(WG), lds, s, mf, alpha, beta, lambda, x, err);
+  return hipGetLastError();
+}
+
+hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hipStream_t s) {
+  hipLaunchKernelGGL(ps_math_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, out, out2);
+  return hipGetLastError();
+}
+
+hipError_t ps_set_fit_attributes(int nb, int nbs, size_t lds) {
+#define PS_A(NB_, NBS_)                                                                                   \
+  if (nb == NB_ && nbs == NBS_) {                                                                         \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_>),           \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);             \
+    if (e == hipSuccess)                                                                                  \
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_>),                   \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                      \
+    return e;                                                                                             \
+  }
+  PS_FOR_EACH_SHAPE(PS_A)
+#undef PS_A
+  return hipErrorInvalidValue;
+}
