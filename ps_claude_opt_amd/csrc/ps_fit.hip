@@ -77,6 +77,36 @@ struct Inv {
   double omega, A, invD, kph, invD2, cl, cls, sb, cb, sl, clam;
 };
 
+__device__ __forceinline__ void store_inv(const Inv& v, double* d) {
+  d[0] = v.omega;
+  d[1] = v.A;
+  d[2] = v.invD;
+  d[3] = v.kph;
+  d[4] = v.invD2;
+  d[5] = v.cl;
+  d[6] = v.cls;
+  d[7] = v.sb;
+  d[8] = v.cb;
+  d[9] = v.sl;
+  d[10] = v.clam;
+}
+
+__device__ __forceinline__ Inv load_inv(const double* d) {
+  Inv v;
+  v.omega = d[0];
+  v.A = d[1];
+  v.invD = d[2];
+  v.kph = d[3];
+  v.invD2 = d[4];
+  v.cl = d[5];
+  v.cls = d[6];
+  v.sb = d[7];
+  v.cb = d[8];
+  v.sl = d[9];
+  v.clam = d[10];
+  return v;
+}
+
 // Pair invariants of one parameter vector a[] (a[i] = parameter i+1).
 __device__ Inv make_inv(const PsDevProblem& P, const double* a) {
   const int C = P.ncoef;
@@ -149,6 +179,7 @@ struct Lds {
   double* x;       // [K] right-hand side / solution
   double* dum;     // [K] elimination multipliers
   double* red;     // [2*NW] reduction scratch
+  double* inv;     // [16] pair invariants of the evaluated parameters
   int* ipiv;       // [K]
   int* ired;       // [2*NW]
 };
@@ -175,6 +206,8 @@ __device__ Lds carve_lds(double* base, const PsDevProblem& P) {
   p += K;
   L.red = p;
   p += 2 * NW;
+  L.inv = p;
+  p += 16;
   L.ipiv = reinterpret_cast<int*>(p);
   p += (K + 1) / 2 + 1;
   L.ired = reinterpret_cast<int*>(p);
@@ -206,9 +239,10 @@ __device__ void curv_block(const PsDevProblem& P, const double* a, double* area)
 
 // Phase 1 for one 16-point tile of one curve, executed by one wave.
 template <int NB, int NBS>
-__device__ unsigned phase1_tile(const PsDevProblem& P, const Inv& v, const double* area, int p0, int lp, int t,
-                                double* jbuf, double* ybuf, double* rowbuf) {
+__device__ unsigned phase1_tile(const PsDevProblem& P, const double* inv, const double* area, int p0, int lp,
+                                int t, double* jbuf, double* ybuf, double* rowbuf) {
   const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
+  const Inv v = load_inv(inv);
   const int F = P.nfac, QS = P.qs, C = P.ncoef;
   const int pin = t * 16 + ps;
   const bool valid = pin < lp;
@@ -345,7 +379,6 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Inv& v, const doubl
 template <int NB>
 __device__ void phase2_tile(const PsDevProblem& P, int p0, int lp, int t, const double* jbuf, const double* ybuf,
                             const double* dvec, double rave, bool conv, double conv_w, v4d* acc) {
-  constexpr int NP = NB * (NB + 1) / 2;
   const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
   const int mf = P.mfit, QS = P.qs, C = P.ncoef;
 #pragma unroll
@@ -394,8 +427,49 @@ __device__ void phase2_tile(const PsDevProblem& P, int p0, int lp, int t, const 
         if (idx % NW == w) acc[idx / NW] = mfma(X[ib], XW[jb], acc[idx / NW]);
         ++idx;
       }
-    (void)NP;
   }
+}
+
+// The augmented normal matrix lives in L.work (lower triangle, stride K)
+// between phase-2 passes; each wave moves its own 16x16 block pairs between
+// LDS and MFMA accumulators, so every element stays one fused multiply-add
+// chain over the points while the accumulators are not live in phase 1.
+template <int NB>
+__device__ __forceinline__ void blocks_load(const double* work, int K, v4d* acc) {
+  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
+  int idx = 0;
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int jb = 0; jb <= ib; ++jb) {
+      if (idx % NW == w) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ai = 16 * ib + 4 * r + sl, bj = 16 * jb + ps;
+          acc[idx / NW][r] = (ai < K && bj <= ai) ? work[ai * K + bj] : 0.0;
+        }
+      }
+      ++idx;
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void blocks_store(double* work, int K, const v4d* acc) {
+  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
+  int idx = 0;
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int jb = 0; jb <= ib; ++jb) {
+      if (idx % NW == w) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ai = 16 * ib + 4 * r + sl, bj = 16 * jb + ps;
+          if (ai < K && bj <= ai) work[ai * K + bj] = acc[idx / NW][r];
+        }
+      }
+      ++idx;
+    }
 }
 
 // One mrqcof evaluation at a[] (LDS) into L.work (K*K, symmetric alpha,
@@ -405,20 +479,18 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
                          double* ybuf, unsigned long long* vis_count) {
   constexpr int NP = NB * (NB + 1) / 2;
   constexpr int NPW = (NP + NW - 1) / NW;
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, ps = lane & 15, sl = lane >> 4;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac;
   curv_block(P, a, L.area);
-  const Inv v = make_inv(P, a);
+  if (tid == 0) store_inv(make_inv(P, a), L.inv);
+  for (int e = tid; e < K * K; e += WG) L.work[e] = 0.0;
   __syncthreads();
-  v4d acc[NPW];
-#pragma unroll
-  for (int i = 0; i < NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
   for (int c = 0; c < P.ncurves; ++c) {
     const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
     const int ntile = (lp + 15) / 16;
-    for (int t = w; t < ntile; t += NW) nvis += phase1_tile<NB, NBS>(P, v, L.area, p0, lp, t, jbuf, ybuf, rowbuf);
+    for (int t = w; t < ntile; t += NW) nvis += phase1_tile<NB, NBS>(P, L.inv, L.area, p0, lp, t, jbuf, ybuf, rowbuf);
     __syncthreads();
     // curve sums, sequential over points (column q = parameter q+2)
     for (int q = tid; q <= mf; q += WG) {
@@ -431,8 +503,13 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
       L.dvec[q] = s;
     }
     __syncthreads();
-    const double rave = 1.0 / L.dvec[mf];
-    for (int t = 0; t < ntile; ++t) phase2_tile<NB>(P, p0, lp, t, jbuf, ybuf, L.dvec, rave, false, 0.0, acc);
+    {
+      const double rave = 1.0 / L.dvec[mf];
+      v4d acc[NPW];
+      blocks_load<NB>(L.work, K, acc);
+      for (int t = 0; t < ntile; ++t) phase2_tile<NB>(P, p0, lp, t, jbuf, ybuf, L.dvec, rave, false, 0.0, acc);
+      blocks_store<NB>(L.work, K, acc);
+    }
     __syncthreads();
   }
   if (vis_count && lane == 0) atomicAdd(vis_count, nvis);
@@ -451,23 +528,11 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
     }
   }
   __syncthreads();
-  phase2_tile<NB>(P, 0, 3, 0, jbuf, ybuf, L.dvec, 0.0, true, conw_r2, acc);
-  // each wave stores its block pairs (lower triangle)
   {
-    int idx = 0;
-#pragma unroll
-    for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-      for (int jb = 0; jb <= ib; ++jb) {
-        if (idx % NW == w) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int ai = 16 * ib + 4 * r + sl, bj = 16 * jb + ps;
-            if (ai < K && bj <= ai) L.work[ai * K + bj] = acc[idx / NW][r];
-          }
-        }
-        ++idx;
-      }
+    v4d acc[NPW];
+    blocks_load<NB>(L.work, K, acc);
+    phase2_tile<NB>(P, 0, 3, 0, jbuf, ybuf, L.dvec, 0.0, true, conw_r2, acc);
+    blocks_store<NB>(L.work, K, acc);
   }
   __syncthreads();
   for (int e = tid; e < mf * mf; e += WG) {
@@ -770,13 +835,13 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
   } else {
     // raw Jacobian rows of every data point: phase 1 only, curve by curve
     curv_block(P, L.cgg, L.area);
-    const Inv v = make_inv(P, L.cgg);
+    if (tid == 0) store_inv(make_inv(P, L.cgg), L.inv);
     __syncthreads();
     const int w = tid >> 6;
     for (int c = 0; c < P.ncurves; ++c) {
       const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
       for (int t = w; t < (lp + 15) / 16; t += NW)
-        phase1_tile<NB, NBS>(P, v, L.area, p0, lp, t, jbuf, ybuf, L.rowbuf + w * 256);
+        phase1_tile<NB, NBS>(P, L.inv, L.area, p0, lp, t, jbuf, ybuf, L.rowbuf + w * 256);
       __syncthreads();
       for (int e = tid; e < lp * (P.ma + 1); e += WG) {
         const int j = e / (P.ma + 1), col = e - j * (P.ma + 1);
@@ -839,7 +904,7 @@ __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2
 // ---------------------------------------------------------------------------
 size_t ps_fit_lds_bytes(const PsDevProblem& P) {
   const int K = P.mfit + 1;
-  size_t d = (size_t)K * K + P.nfac + 2 * P.ma + P.qs + NW * 256 + 2 * K + 2 * NW + (K + 1) / 2 + 1 + 2 * NW;
+  size_t d = (size_t)K * K + P.nfac + 2 * P.ma + P.qs + NW * 256 + 2 * K + 2 * NW + 16 + (K + 1) / 2 + 1 + 2 * NW;
   return d * sizeof(double) + 64;
 }
 
