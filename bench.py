@@ -48,11 +48,16 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--preset", default="c2")
-    ap.add_argument("--cpu-pairs", type=int, default=32)
+    ap.add_argument("--cpu-pairs", type=int, default=128)
     ap.add_argument("--max-freqs", type=int, default=0, help="scan only the first N frequencies (0 = all)")
+    ap.add_argument("--conw-r", type=float, default=0.0, help="use this conwR instead of searching for it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--work-dir", default=os.path.join(ROOT, "gpurun_out", "bench"))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; pairs shard across GPUs by frequency "
               "(see DESIGN.md), but multi-GPU runs are not part of it.", file=sys.stderr)
@@ -77,7 +82,7 @@ def main() -> int:
     if args.max_freqs > 0:
         n_max = min(n_max, args.max_freqs)
     t0 = time.perf_counter()
-    conw_r = scan.conw_search()
+    conw_r = args.conw_r if args.conw_r > 0 else scan.conw_search()
     t_conw = time.perf_counter() - t0
     print(f"# conw search: conwR {conw_r:.6g} in {t_conw:.2f} s", file=sys.stderr, flush=True)
 
@@ -156,6 +161,23 @@ def main() -> int:
         result["cpu_baseline"] = cpu_baseline(path, conw_r, n_max, args.cpu_pairs)
     print(json.dumps(result), flush=True)
     return 0
+
+
+def copy_probe(mib: int) -> int:
+    """Five device-to-device copies of a known size.  Under a counter
+    profiler their FETCH_SIZE / WRITE_SIZE per dispatch give the factor
+    between the counters and the bytes a kernel really moves."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(x, y))
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "ok": ok}), flush=True)
+    return 0 if ok else 1
 
 
 def cpu_baseline(path: str, conw_r: float, n_max: int, n_pairs: int) -> dict:

@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -256,7 +257,8 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   const int nb = std::max((C + 7 + 15) / 16, (K + 15) / 16);
   if (nb > 6 || (C - 1 + 15) / 16 < nb - 1) return fail(c, PS_E_UNSUPPORTED, "more than 96 Jacobian columns (lmax too large)");
   // host-derived tables
-  std::vector<double> alph(N), s2w(N), yg((size_t)F * 16 * nb, 0.0);
+  const int nbs = (C - 1 + 15) / 16;
+  std::vector<double> alph(N), s2w(N), yl((size_t)F * 16 * nbs, 0.0);
   for (int i = 0; i < N; ++i) {
     const double d = std::fma(p->ee[2 * N + i], p->ee0[2 * N + i],
                               std::fma(p->ee[N + i], p->ee0[N + i], p->ee[i] * p->ee0[i]));
@@ -264,7 +266,7 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
     s2w[i] = p->sigr2[i] * p->weight[i];
   }
   for (int f = 0; f < F; ++f)
-    for (int q = 0; q <= C - 2; ++q) yg[(size_t)f * 16 * nb + q] = p->d_sphere[(size_t)(q + 1) * F + f];
+    for (int q = 0; q <= C - 2; ++q) yl[((size_t)f * 16 + (q & 15)) * nbs + (q >> 4)] = p->d_sphere[(size_t)(q + 1) * F + f];
   std::vector<int> off(L + 1, 0), cmap;
   int maxlp = 3;
   for (int i = 0; i < L; ++i) {
@@ -297,7 +299,7 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   UP(pleg, p->pleg, (size_t)(p->mmax + 1) * (p->lmax + 1) * F);
   UP(fc, p->fc, (size_t)(p->mmax + 1) * F);
   UP(fs, p->fs, (size_t)(p->mmax + 1) * F);
-  UP(yg, yg.data(), yg.size());
+  UP(yl, yl.data(), yl.size());
   UP(cg_first, p->cg_first, C);
   UP(cmap, cmap.data(), cmap.size());
 #undef UP
@@ -311,7 +313,7 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   d.mfit = mfit;
   d.nb = nb;
   d.qs = 16 * nb;
-  d.nbs = (C - 1 + 15) / 16;
+  d.nbs = nbs;
   for (int i = 0; i < 3; ++i) d.par[i] = p->par[i];
   d.lcl = std::log(p->cl);
   d.phi0 = p->phi0;
@@ -353,12 +355,11 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   c->max_lp = maxlp;
   c->jrows = ((maxlp + 15) / 16) * 16;
   c->stride = (long long)ps_fit_scratch_stride(d, c->jrows);
-  c->lds = ps_fit_lds_bytes(d);
-  hipDeviceProp_t prop;
-  PS_HIP(c, hipGetDeviceProperties(&prop, c->device));
-  if (c->lds > (size_t)prop.sharedMemPerBlock && c->lds > 160 * 1024)
-    return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
-  PS_HIP(c, ps_set_fit_attributes(nb, d.nbs, c->lds));
+  const size_t lds_cap = 160 * 1024;
+  d.ylds = ps_fit_lds_bytes(d, true) <= lds_cap ? 1 : 0;
+  c->lds = ps_fit_lds_bytes(d, d.ylds != 0);
+  if (c->lds > lds_cap) return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
+  PS_HIP(c, ps_set_fit_attributes(nb, d.nbs, d.ylds != 0, c->lds));
   const int per_cu = std::max(1, std::min(4, (int)((160 * 1024) / c->lds)));
   c->grid_cap = c->n_cu * per_cu;
   c->prepared = true;
@@ -422,6 +423,14 @@ int ps_fit_pairs(ps_ctx* c, int npairs, const int* n_list, const int* m_list, do
   PS_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   c->last_launches = 1;
+  if (std::getenv("PS_PHASE_TIMING")) {
+    unsigned long long cyc[16];
+    if (ps_phase_cycles(cyc, true) == hipSuccess) {
+      std::fprintf(stderr, "phase cycles (thread 0 of each workgroup, summed):");
+      for (int i = 0; i < 9; ++i) std::fprintf(stderr, " %d:%.4g", i, (double)cyc[i]);
+      std::fprintf(stderr, "\n");
+    }
+  }
   for (int i = 0; i < npairs; ++i) {
     const double* o = &res[(size_t)i * PS_OUT_STRIDE];
     ps_pair_result& r = out[i];

@@ -36,6 +36,30 @@
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+#ifdef PS_PHASE_TIMING
+// Debug builds only: cycles spent per phase by thread 0 of every workgroup
+// (including its waits at the phase's closing barrier).
+__device__ unsigned long long g_phase_cycles[16];
+#define PS_TSTART() unsigned long long ps_t0_ = clock64()
+#define PS_TMARK(i)                                                  \
+  do {                                                               \
+    if (threadIdx.x == 0) {                                          \
+      const unsigned long long t1_ = clock64();                      \
+      atomicAdd(&g_phase_cycles[i], t1_ - ps_t0_);                   \
+      ps_t0_ = t1_;                                                  \
+    } else {                                                         \
+      ps_t0_ = clock64();                                            \
+    }                                                                \
+  } while (0)
+#else
+#define PS_TSTART() \
+  do {              \
+  } while (0)
+#define PS_TMARK(i) \
+  do {              \
+  } while (0)
+#endif
+
 namespace {
 
 constexpr int NW = PS_FIT_WAVES;  // waves per workgroup
@@ -171,7 +195,8 @@ __device__ __forceinline__ void point_geometry(const PsDevProblem& P, const Inv&
 // LDS layout of one workgroup (dynamic shared memory).
 struct Lds {
   double* work;    // [K*K] normal matrix target / Gauss-Jordan matrix
-  double* area;    // [F]
+  double* nrm;     // [F][4] facet normal x, y, z and the current facet area
+  double* ytab;    // [F][16][nbs] shape basis (LDS copy, or the global table)
   double* cgg;     // [ma]
   double* atry;    // [ma]
   double* dvec;    // [qs] column sums of the current curve
@@ -184,14 +209,20 @@ struct Lds {
   int* ired;       // [2*NW]
 };
 
-__device__ Lds carve_lds(double* base, const PsDevProblem& P) {
+__device__ Lds carve_lds(double* base, const PsDevProblem& P, bool ylds) {
   Lds L;
   const int K = P.mfit + 1;
   double* p = base;
   L.work = p;
   p += K * K;
-  L.area = p;
-  p += P.nfac;
+  L.nrm = p;
+  p += 4 * P.nfac;
+  if (ylds) {
+    L.ytab = p;
+    p += (size_t)P.nfac * 16 * P.nbs;
+  } else {
+    L.ytab = const_cast<double*>(P.yl);
+  }
   L.cgg = p;
   p += P.ma;
   L.atry = p;
@@ -214,8 +245,30 @@ __device__ Lds carve_lds(double* base, const PsDevProblem& P) {
   return L;
 }
 
-// Facet areas of parameter vector a (Start.cu:421-528).
-__device__ void curv_block(const PsDevProblem& P, const double* a, double* area) {
+// Copies the facet normals (and the basis table when it lives in LDS) into
+// the workgroup's LDS once per launch.
+__device__ void stage_tables(const PsDevProblem& P, const Lds& L, bool ylds) {
+  const int F = P.nfac;
+  for (int f = threadIdx.x; f < F; f += WG) {
+    L.nrm[4 * f + 0] = P.nx[f];
+    L.nrm[4 * f + 1] = P.ny[f];
+    L.nrm[4 * f + 2] = P.nz[f];
+    L.nrm[4 * f + 3] = 0.0;
+  }
+  if (ylds) {
+    const int n = F * 16 * P.nbs;
+    for (int i = threadIdx.x; i < n; i += WG) L.ytab[i] = P.yl[i];
+  }
+  __syncthreads();
+}
+
+// Basis value of shape column q (parameter q+2) at facet f.
+__device__ __forceinline__ double ybasis(const PsDevProblem& P, const double* ytab, int f, int q) {
+  return ytab[((size_t)f * 16 + (q & 15)) * P.nbs + (q >> 4)];
+}
+
+// Facet areas of parameter vector a (Start.cu:421-528) into nrm[4f+3].
+__device__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) {
   const int F = P.nfac, Lm = P.lmax, M = P.mmax;
   for (int f = threadIdx.x; f < F; f += WG) {
     double g = 0.0;
@@ -233,20 +286,21 @@ __device__ void curv_block(const PsDevProblem& P, const double* a, double* area)
         g = fma(P.pleg[(m * (Lm + 1) + l) * F + f], fsum, g);
       }
     }
-    area[f] = P.darea[f] * psm::exp(g);
+    nrm[4 * f + 3] = P.darea[f] * psm::exp(g);
   }
 }
 
 // Phase 1 for one 16-point tile of one curve, executed by one wave.
 template <int NB, int NBS>
-__device__ unsigned phase1_tile(const PsDevProblem& P, const double* inv, const double* area, int p0, int lp,
-                                int t, double* jbuf, double* ybuf, double* rowbuf) {
+__device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int lp, int t, double* jbuf,
+                                double* ybuf, double* rowbuf) {
   const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
-  const Inv v = load_inv(inv);
+  const Inv v = load_inv(L.inv);
   const int F = P.nfac, QS = P.qs, C = P.ncoef;
   const int pin = t * 16 + ps;
   const bool valid = pin < lp;
   const int pt = p0 + (valid ? pin : 0);
+  (void)NB;
   // point geometry: only the body-frame directions stay live in the loop
   double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0}, scr[3];
   double sf = 0.0, cf = 0.0, scale = 0.0, ff = 0.0, d2 = 0.0, alph = 0.0;
@@ -268,14 +322,18 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const double* inv, const 
 #pragma unroll
   for (int b = 0; b < NBS; ++b) gat[b] = v4d{0.0, 0.0, 0.0, 0.0};
   const double cl = v.cl, cls = v.cls;
-  const double* __restrict__ nxp = P.nx;
-  const double* __restrict__ nyp = P.ny;
-  const double* __restrict__ nzp = P.nz;
-  const double* __restrict__ yg = P.yg;
+  // lane facet f = 4k + sl: normal/area at nrm[4f..4f+3], basis values of
+  // shape columns 16b + ps at ytab[(16f + ps) * NBS + b]
+  const double* __restrict__ nr = L.nrm + 4 * sl;
+  const double* __restrict__ yb = L.ytab + (size_t)(16 * sl + ps) * NBS;
   unsigned nvis = 0;
   for (int k = 0; k < F / 4; ++k) {
-    const int f = 4 * k + sl;
-    const double n0 = nxp[f], n1 = nyp[f], n2 = nzp[f];
+    const double* nrk = nr + 16 * k;
+    const double n0 = nrk[0], n1 = nrk[1], n2 = nrk[2], ar = nrk[3];
+    const double* ybk = yb + (size_t)k * 64 * NBS;
+    double yv[NBS];
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) yv[b] = ybk[b];
     const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
     const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
     const bool vis = (mu > kTiny) && (mu0 > kTiny);
@@ -284,7 +342,6 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const double* inv, const 
     nvis += (unsigned)__popcll(bal);
     double w = 0.0;
     if (vis) {
-      const double ar = area[f];
       const double rd = recip(mu + mu0);
       const double mm = mu * mu0;
       const double sc = mm * fma(cls, rd, cl);
@@ -303,9 +360,8 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const double* inv, const 
       t4 = fma(ar, mm, t4);
       t5 = fma(ar * mm, rd, t5);
     }
-    const double* yrow = yg + (size_t)f * QS + ps;
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, yrow[16 * b], gat[b]);
+    for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, yv[b], gat[b]);
   }
   br = xsum16_32(br);
   t4 = xsum16_32(t4);
@@ -473,7 +529,7 @@ __device__ __forceinline__ void blocks_store(double* work, int K, const v4d* acc
 }
 
 // One mrqcof evaluation at a[] (LDS) into L.work (K*K, symmetric alpha,
-// beta in row mfit, chi^2 at [mfit][mfit]); L.area = curv(a).
+// beta in row mfit, chi^2 at [mfit][mfit]); facet areas of a in L.nrm[4f+3].
 template <int NB, int NBS>
 __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, double* jbuf,
                          double* ybuf, unsigned long long* vis_count) {
@@ -481,17 +537,21 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
   constexpr int NPW = (NP + NW - 1) / NW;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac;
-  curv_block(P, a, L.area);
+  PS_TSTART();
+  curv_block(P, a, L.nrm);
   if (tid == 0) store_inv(make_inv(P, a), L.inv);
   for (int e = tid; e < K * K; e += WG) L.work[e] = 0.0;
   __syncthreads();
+  PS_TMARK(0);
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
   for (int c = 0; c < P.ncurves; ++c) {
     const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
     const int ntile = (lp + 15) / 16;
-    for (int t = w; t < ntile; t += NW) nvis += phase1_tile<NB, NBS>(P, L.inv, L.area, p0, lp, t, jbuf, ybuf, rowbuf);
+    for (int t = w; t < ntile; t += NW) nvis += phase1_tile<NB, NBS>(P, L, p0, lp, t, jbuf, ybuf, rowbuf);
+    PS_TMARK(1);
     __syncthreads();
+    PS_TMARK(2);
     // curve sums, sequential over points (column q = parameter q+2)
     for (int q = tid; q <= mf; q += WG) {
       double s = 0.0;
@@ -503,6 +563,7 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
       L.dvec[q] = s;
     }
     __syncthreads();
+    PS_TMARK(3);
     {
       const double rave = 1.0 / L.dvec[mf];
       v4d acc[NPW];
@@ -510,20 +571,21 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
       for (int t = 0; t < ntile; ++t) phase2_tile<NB>(P, p0, lp, t, jbuf, ybuf, L.dvec, rave, false, 0.0, acc);
       blocks_store<NB>(L.work, K, acc);
     }
+    PS_TMARK(4);
     __syncthreads();
+    PS_TMARK(5);
   }
   if (vis_count && lane == 0) atomicAdd(vis_count, nvis);
   // convexity pseudo-curve (Start.cu:2539-2612): y_j = sum Area n_j,
   // J_j[c] = sum (Area n_j) Y_c
   for (int idx = tid; idx < 3 * C; idx += WG) {
     const int j = idx / C, c = idx - j * C;
-    const double* nj = (j == 0) ? P.nx : (j == 1) ? P.ny : P.nz;
     double s = 0.0;
     if (c == 0) {
-      for (int f = 0; f < F; ++f) s = s + L.area[f] * nj[f];
+      for (int f = 0; f < F; ++f) s = s + L.nrm[4 * f + 3] * L.nrm[4 * f + j];
       ybuf[j] = s;
     } else {
-      for (int f = 0; f < F; ++f) s = fma(L.area[f] * nj[f], P.yg[(size_t)f * QS + (c - 1)], s);
+      for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3] * L.nrm[4 * f + j], ybasis(P, L.ytab, f, c - 1), s);
       jbuf[(size_t)j * QS + (c - 1)] = s;
     }
   }
@@ -540,6 +602,7 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
     if (j > i) L.work[i * K + j] = L.work[j * K + i];
   }
   __syncthreads();
+  PS_TMARK(6);
 }
 
 // Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) on L.work (stride K,
@@ -657,13 +720,14 @@ struct PairState {
 // ---------------------------------------------------------------------------
 // Persistent LM fit kernel.
 // ---------------------------------------------------------------------------
-template <int NB, int NBS>
+template <int NB, int NBS, bool YLDS>
 __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
   extern __shared__ double lds_dyn[];
   __shared__ PairState S;
   __shared__ int item_s;
   const PsDevProblem& P = A.P;
-  const Lds L = carve_lds(lds_dyn, P);
+  const Lds L = carve_lds(lds_dyn, P, YLDS);
+  stage_tables(P, L, YLDS);
   const int tid = threadIdx.x;
   const int mf = P.mfit, K = mf + 1, C = P.ncoef, ma = P.ma, F = P.nfac;
   double* jbuf = A.scratch + (size_t)blockIdx.x * A.scratch_stride;
@@ -714,6 +778,7 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
 #pragma unroll 1
       for (int pass = is_alambda ? 0 : 1; pass < 2; ++pass) {
         if (pass == 1) {
+          PS_TSTART();
           const double ccc = 1 + S.alamda;
           for (int e = tid; e < mf * mf; e += WG) {
             const int i = e / mf, j = e - i * mf;
@@ -728,6 +793,7 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
           if (err == 0 || err == 2)
             for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.x[q];
           __syncthreads();
+          PS_TMARK(7);
         }
         evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, A.conw_r2_rows, jbuf, ybuf, &S.nvis);
         if (pass == 0) {
@@ -757,13 +823,13 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
       }
       __syncthreads();
       // Iter2 (Start.cu:5129-5248)
+      PS_TSTART();
       if (S.niter == 1 || S.chisq < S.ochisq) {
-        curv_block(P, L.cgg, L.area);
+        curv_block(P, L.cgg, L.nrm);
         __syncthreads();
         if (tid < 3) {
-          const double* nj = (tid == 0) ? P.nx : (tid == 1) ? P.ny : P.nz;
           double s = 0.0;
-          for (int f = 0; f < F; ++f) s = fma(L.area[f], nj[f], s);
+          for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3], L.nrm[4 * f + tid], s);
           L.red[tid] = s;
         }
         __syncthreads();
@@ -783,6 +849,7 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
         }
       }
       __syncthreads();
+      PS_TMARK(8);
     }
     // FinishPole (Start.cu:5056-5125)
     if (tid == 0) {
@@ -790,7 +857,7 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
       double dev = 1e40, dark = 0, per = 0, la = 0, be = 0;
       if (!(S.dev_new >= 1e40)) {
         double tot = 0.0;
-        for (int f = 0; f < F; ++f) tot = tot + L.area[f];
+        for (int f = 0; f < F; ++f) tot = tot + L.nrm[4 * f + 3];
         tot = 1.0 / tot;
         double dd = (S.chck * 100.0) * tot;
         if (dd != dd) dd = 1.0;
@@ -819,12 +886,13 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
 // ---------------------------------------------------------------------------
 // Single evaluation / Gauss-Jordan / math kernels for the parity tests.
 // ---------------------------------------------------------------------------
-template <int NB, int NBS>
+template <int NB, int NBS, bool YLDS>
 __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* a, double* out_work,
                                                      double* out_area, double* out_j) {
   extern __shared__ double lds_dyn[];
   const PsDevProblem& P = A.P;
-  const Lds L = carve_lds(lds_dyn, P);
+  const Lds L = carve_lds(lds_dyn, P, YLDS);
+  stage_tables(P, L, YLDS);
   const int tid = threadIdx.x, K = P.mfit + 1;
   double* jbuf = A.scratch;
   double* ybuf = jbuf + (size_t)A.jrows * P.qs;
@@ -834,14 +902,14 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
     evaluate<NB, NBS>(P, L, L.cgg, A.conw_r2_rows, jbuf, ybuf, nullptr);
   } else {
     // raw Jacobian rows of every data point: phase 1 only, curve by curve
-    curv_block(P, L.cgg, L.area);
+    curv_block(P, L.cgg, L.nrm);
     if (tid == 0) store_inv(make_inv(P, L.cgg), L.inv);
     __syncthreads();
     const int w = tid >> 6;
     for (int c = 0; c < P.ncurves; ++c) {
       const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
       for (int t = w; t < (lp + 15) / 16; t += NW)
-        phase1_tile<NB, NBS>(P, L.inv, L.area, p0, lp, t, jbuf, ybuf, L.rowbuf + w * 256);
+        phase1_tile<NB, NBS>(P, L, p0, lp, t, jbuf, ybuf, L.rowbuf + w * 256);
       __syncthreads();
       for (int e = tid; e < lp * (P.ma + 1); e += WG) {
         const int j = e / (P.ma + 1), col = e - j * (P.ma + 1);
@@ -858,7 +926,7 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
     }
   }
   for (int e = tid; e < K * K; e += WG) out_work[e] = L.work[e];
-  for (int f = tid; f < P.nfac; f += WG) out_area[f] = L.area[f];
+  for (int f = tid; f < P.nfac; f += WG) out_area[f] = L.nrm[4 * f + 3];
 }
 
 __global__ __launch_bounds__(WG) void ps_gaussj_kernel(int mf, const double* alpha, const double* beta, double lambda,
@@ -902,9 +970,10 @@ __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-size_t ps_fit_lds_bytes(const PsDevProblem& P) {
+size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds) {
   const int K = P.mfit + 1;
-  size_t d = (size_t)K * K + P.nfac + 2 * P.ma + P.qs + NW * 256 + 2 * K + 2 * NW + 16 + (K + 1) / 2 + 1 + 2 * NW;
+  size_t d = (size_t)K * K + 4 * P.nfac + 2 * P.ma + P.qs + NW * 256 + 2 * K + 2 * NW + 16 + (K + 1) / 2 + 1 + 2 * NW;
+  if (ylds) d += (size_t)P.nfac * 16 * P.nbs;
   return d * sizeof(double) + 64;
 }
 
@@ -915,15 +984,20 @@ size_t ps_fit_scratch_stride(const PsDevProblem& P, int jrows) {
 }
 
 // (NB, NBS) instantiations: NB = 16-column blocks of a Jacobian row, NBS =
-// blocks holding shape columns (NBS in {NB-1, NB}).
+// blocks holding shape columns (NBS in {NB-1, NB}); YLDS = basis table in
+// LDS (when it fits next to the rest of the workgroup's LDS).
 #define PS_FOR_EACH_SHAPE(X) X(1, 1) X(2, 1) X(2, 2) X(3, 2) X(3, 3) X(4, 3) X(4, 4) X(5, 4) X(5, 5) X(6, 5) X(6, 6)
 
 hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s) {
-  const size_t lds = ps_fit_lds_bytes(A.P);
-#define PS_L(NB_, NBS_)                                                                                   \
-  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                 \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_fit_kernel<NB_, NBS_>), dim3(grid), dim3(WG), lds, s, A);       \
-    return hipGetLastError();                                                                             \
+  const bool yl = A.P.ylds != 0;
+  const size_t lds = ps_fit_lds_bytes(A.P, yl);
+#define PS_L(NB_, NBS_)                                                                                       \
+  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                     \
+    if (yl)                                                                                                   \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_fit_kernel<NB_, NBS_, true>), dim3(grid), dim3(WG), lds, s, A);   \
+    else                                                                                                      \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_fit_kernel<NB_, NBS_, false>), dim3(grid), dim3(WG), lds, s, A);  \
+    return hipGetLastError();                                                                                 \
   }
   PS_FOR_EACH_SHAPE(PS_L)
 #undef PS_L
@@ -932,16 +1006,36 @@ hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s) {
 
 hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
                           double* jrows) {
-  const size_t lds = ps_fit_lds_bytes(A.P);
-#define PS_L(NB_, NBS_)                                                                                   \
-  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                 \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_>), dim3(1), dim3(WG), lds, s, A, a, work,  \
-                       area, jrows);                                                                      \
-    return hipGetLastError();                                                                             \
+  const bool yl = A.P.ylds != 0;
+  const size_t lds = ps_fit_lds_bytes(A.P, yl);
+#define PS_L(NB_, NBS_)                                                                                       \
+  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                     \
+    if (yl)                                                                                                   \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_, true>), dim3(1), dim3(WG), lds, s, A, a,  \
+                         work, area, jrows);                                                                  \
+    else                                                                                                      \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_, false>), dim3(1), dim3(WG), lds, s, A, a, \
+                         work, area, jrows);                                                                  \
+    return hipGetLastError();                                                                                 \
   }
   PS_FOR_EACH_SHAPE(PS_L)
 #undef PS_L
   return hipErrorInvalidValue;
+}
+
+hipError_t ps_phase_cycles(unsigned long long* out, bool reset) {
+#ifdef PS_PHASE_TIMING
+  hipError_t e = hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 16);
+  if (e == hipSuccess && reset) {
+    unsigned long long z[16] = {0};
+    e = hipMemcpyToSymbol(HIP_SYMBOL(g_phase_cycles), z, sizeof z);
+  }
+  return e;
+#else
+  (void)reset;
+  for (int i = 0; i < 16; ++i) out[i] = 0;
+  return hipSuccess;
+#endif
 }
 
 hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda, double* x, int* err,
@@ -957,15 +1051,16 @@ hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hip
   return hipGetLastError();
 }
 
-hipError_t ps_set_fit_attributes(int nb, int nbs, size_t lds) {
-#define PS_A(NB_, NBS_)                                                                                   \
-  if (nb == NB_ && nbs == NBS_) {                                                                         \
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_>),           \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);             \
-    if (e == hipSuccess)                                                                                  \
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_>),                   \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                      \
-    return e;                                                                                             \
+hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds) {
+#define PS_A(NB_, NBS_)                                                                                       \
+  if (nb == NB_ && nbs == NBS_) {                                                                             \
+    const void* fk = ylds ? reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, true>)                    \
+                          : reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, false>);                  \
+    const void* ek = ylds ? reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_, true>)                   \
+                          : reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_, false>);                 \
+    hipError_t e = hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);              \
+    if (e == hipSuccess) e = hipFuncSetAttribute(ek, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
+    return e;                                                                                                 \
   }
   PS_FOR_EACH_SHAPE(PS_A)
 #undef PS_A

@@ -23,7 +23,8 @@ struct PsDevProblem {
   int nbs;  // blocks holding shape columns
   const double *nx, *ny, *nz, *darea;
   const double *pleg, *fc, *fs;  // curv tables
-  const double* yg;              // [F][qs]: column q = basis function of parameter q+2
+  const double* yl;              // [F][16][nbs]: shape column q = 16b + r (parameter q+2) at [f][r][b]
+  int ylds;                      // 1: the fit kernel copies yl into LDS
   const double* cg_first;        // [ncoef]
   const int* cmap;               // [mfit] compact index -> parameter index (0-based)
   // model and LM controls
@@ -49,12 +50,13 @@ struct PsFitArgs {
   double* out;          // [n_items][PS_OUT_STRIDE]
 };
 
-size_t ps_fit_lds_bytes(const PsDevProblem& P);
+size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds);
 size_t ps_fit_scratch_stride(const PsDevProblem& P, int jrows);
-hipError_t ps_set_fit_attributes(int nb, int nbs, size_t lds);
+hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds);
 hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s);
 hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
                           double* jrows);
 hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda, double* x, int* err,
                             hipStream_t s);
+hipError_t ps_phase_cycles(unsigned long long* out, bool reset);
 hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hipStream_t s);
