@@ -32,6 +32,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LM pair-fits/sec on 1 MI355X (FP64, warm vs plain) + % FP64 peak, same poles"
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+# HBM bytes per pair-fit of the fit kernel, from FETCH_SIZE / WRITE_SIZE
+# counter passes scaled by a plain copy of known size.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ps_fit_kernel_traffic.json")
+
+
+def measured_traffic(preset: str, pairs_per_launch: float):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            e = json.load(f).get(preset)
+    except (OSError, ValueError):
+        return None
+    if not e:
+        return None
+    return round((e["fetch_bytes_per_pair"] + e["write_bytes_per_pair"]) * pairs_per_launch)
 
 
 def _cores() -> int:
@@ -146,7 +160,8 @@ def main() -> int:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.preset, pairs / steps),
+            "traffic_unit": "bytes per launch (counter passes, profiles/ps_fit_kernel_traffic.json)",
             "kernel": "ps_fit_kernel",
             "kernel_ms_per_launch": round(float(np.mean(kern_ms)), 3),
             "gflop_per_pair_fit": round(total_flops / pairs / 1e9, 4),
