@@ -36,10 +36,19 @@
  *     combined as (P0 + P1) + (P2 + P3);
  *   - shape-Jacobian gather: one fused multiply-add chain over facets in
  *     ascending order (invisible facets add an exact zero);
- *   - column sums / curve brightness sums: sequential over points;
- *   - J^T W J, J^T W r and chi^2: one augmented product [J'|r]^T W [J'|r];
- *     every element is one fused multiply-add chain over all points in
- *     order (data curves in file order, then the three convexity rows).
+ *   - points are grouped in tiles of 16 consecutive points of one curve
+ *     (aligned at the curve start; the last tile of a curve is padded);
+ *     tiles are numbered over all curves in file order, the three
+ *     convexity rows form one more tile after the last data tile;
+ *   - column sums / curve brightness sums: per tile four partial sums over
+ *     the tile's points p = s (mod 4) in order (padding adds +0.0),
+ *     combined as (G0 + G1) + (G2 + G3); the curve sum adds the tile sums
+ *     sequentially;
+ *   - J^T W J, J^T W r and chi^2: one augmented product [J'|r]^T W [J'|r]
+ *     split into G chains (G = 4 for rows of at most 64 columns, else 2):
+ *     chain g holds the tiles T = g (mod G), every element a fused
+ *     multiply-add chain over its tiles' points in order; the chains are
+ *     combined as (C0 + C1) + (C2 + C3) (G = 4) or C0 + C1 (G = 2).
  * Transcendentals (exp, exp2, sin/cos of the LM parameters) use this file's
  * own Cody-Waite + Taylor implementations, which the kernel restates
  * identically; host-side setup uses the C library as the reference does.
@@ -808,6 +817,18 @@ struct EvalOut {
   long long nvis = 0;
 };
 
+/* 16-column blocks of a Jacobian row as the device lays it out: the row
+ * holds parameters 2..C+8 and the augmented normal matrix K = mfit + 1
+ * columns. */
+int row_blocks(const Problem& P) {
+  const int a = (P.ncoef + 7 + 15) / 16, b = (P.mfit + 1 + 15) / 16;
+  return a > b ? a : b;
+}
+
+/* Number of point chains of the normal-equation product (one per group of
+ * waves that share the column blocks). */
+int tile_chains(int nb) { return nb <= 4 ? 4 : 2; }
+
 /* One mrqcof evaluation (Start.cu:4464-4498, 2262-2828, 4791-4807):
  * alpha (mfit x mfit, symmetric), beta (mfit), chisq; area[] = curv(a).
  * Relative curves only on the GPU path; absolute data curves are handled by
@@ -818,24 +839,27 @@ EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order
                double* alpha, double* beta, double* chisq, double* area, double* jrows_out) {
   EvalOut out;
   const int ma = P.ma, mf = P.mfit, K = mf + 1; /* augmented width */
-  const int TL = ord.tile > 0 ? ord.tile : 16; /* staging only: order-neutral */
+  const int TL = 16; /* tile of 16 points: fixes the chain of every point */
+  (void)ord;
   curv(P, a, area);
   PairInv v = pair_invariants(P, a);
   /* compact column map: compact q (0..mf-1) -> parameter index */
   std::vector<int> cmap;
   for (int c = 1; c <= ma; c++)
     if (P.ia[c]) cmap.push_back(c);
-  std::vector<double> acc((size_t)K * K, 0.0); /* lower triangle incl. residual row */
+  const int G = tile_chains(row_blocks(P));
+  std::vector<double> acc((size_t)G * K * K, 0.0); /* G chains, lower triangle incl. residual row */
   std::vector<double> rows((size_t)(P.max_lp > 3 ? P.max_lp : 3) * (ma + 1));
   std::vector<double> X((size_t)TL * K), Xw((size_t)TL * K);
-  int pt0 = 0;
+  int pt0 = 0, tile = 0;
   auto syrk_tile = [&](int np, const double* Xt, const double* s2w) {
-    double* A = acc.data();
+    double* A = acc.data() + (size_t)(tile % G) * K * K;
     for (int p = 0; p < np; p++)
       for (int i = 0; i < K; i++) {
         const double xi = Xt[p * K + i];
         for (int j = 0; j <= i; j++) A[i * K + j] = std::fma(xi, s2w[p] * Xt[p * K + j], A[i * K + j]);
       }
+    tile++;
   };
   std::vector<double> s2w(TL);
   for (int ci = 0; ci < P.l_curves; ci++) {
@@ -845,14 +869,22 @@ EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order
       if (jrows_out) std::memcpy(jrows_out + (size_t)(pt0 + j) * (ma + 1), &rows[(size_t)j * (ma + 1)], sizeof(double) * (ma + 1));
     }
     if (P.in_rel[ci]) {
-      /* curve sums, sequential over points (Start.cu:2454-2531) */
+      /* curve sums (Start.cu:2454-2531) in tile order: per 16-point tile,
+         partial sums over points p = s (mod 4), (G0+G1)+(G2+G3), then the
+         tile sums in order */
       std::vector<double> d(ma + 1, 0.0);
-      double Lsum = 0;
-      for (int j = 0; j < lp; j++) {
-        const double* r = &rows[(size_t)j * (ma + 1)];
-        Lsum = Lsum + r[0];
-        for (int c = 2; c <= ma; c++) d[c] = d[c] + r[c];
+      for (int t0 = 0; t0 < lp; t0 += 16) {
+        for (int c = 0; c <= ma; c++) {
+          if (c == 1) continue;
+          double g4[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int p = 0; p < 16; p++) {
+            const double val = (t0 + p < lp) ? rows[(size_t)(t0 + p) * (ma + 1) + c] : 0.0;
+            g4[p & 3] = g4[p & 3] + val;
+          }
+          d[c] = d[c] + ((g4[0] + g4[1]) + (g4[2] + g4[3]));
+        }
       }
+      const double Lsum = d[0];
       const double rave = 1.0 / Lsum;
       for (int t0 = 0; t0 < lp; t0 += TL) {
         int np = lp - t0 < TL ? lp - t0 : TL;
@@ -900,7 +932,12 @@ EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order
     }
     syrk_tile(3, X.data(), s2w.data());
   }
-  const std::vector<double>& tot = acc;
+  std::vector<double> tot((size_t)K * K);
+  for (size_t e = 0; e < (size_t)K * K; e++) {
+    const double* c0 = acc.data() + e;
+    const size_t KK = (size_t)K * K;
+    tot[e] = (G == 4) ? (c0[0] + c0[KK]) + (c0[2 * KK] + c0[3 * KK]) : (G == 2) ? c0[0] + c0[KK] : c0[0];
+  }
   for (int i = 0; i < mf; i++)
     for (int j = 0; j <= i; j++) {
       alpha[i * mf + j] = tot[i * K + j];

@@ -45,7 +45,6 @@ struct ps_ctx {
   PsDevProblem dp{};
   std::vector<void*> dbufs;
   size_t lds = 0;
-  int jrows = 0;
   long long stride = 0;
   int grid_cap = 0;
   // batch buffers
@@ -134,7 +133,6 @@ PsFitArgs make_args(ps_ctx* c, double conw_r, double conw_r_dev, int n_items) {
   A.counter = c->counter;
   A.scratch = c->scratch;
   A.scratch_stride = c->stride;
-  A.jrows = c->jrows;
   A.out = c->d_out;
   return A;
 }
@@ -267,11 +265,14 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   }
   for (int f = 0; f < F; ++f)
     for (int q = 0; q <= C - 2; ++q) yl[((size_t)f * 16 + (q & 15)) * nbs + (q >> 4)] = p->d_sphere[(size_t)(q + 1) * F + f];
-  std::vector<int> off(L + 1, 0), cmap;
+  std::vector<int> off(L + 1, 0), toff(L + 1, 0), tcv, cmap;
   int maxlp = 3;
   for (int i = 0; i < L; ++i) {
     off[i + 1] = off[i] + p->l_points[i];
     maxlp = std::max(maxlp, p->l_points[i]);
+    const int nt = (p->l_points[i] + 15) / 16;
+    toff[i + 1] = toff[i] + nt;
+    for (int t = 0; t < nt; ++t) tcv.push_back(i);
   }
   for (int i = 0; i < ma; ++i)
     if (p->ia[i]) cmap.push_back(i);
@@ -292,6 +293,8 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   UP(sig, p->sig, N);
   UP(s2w, s2w.data(), N);
   UP(curve_off, off.data(), L + 1);
+  UP(tile_off, toff.data(), L + 1);
+  UP(tile_cv, tcv.data(), tcv.size());
   UP(nx, p->normal, F);
   UP(ny, p->normal + F, F);
   UP(nz, p->normal + 2 * F, F);
@@ -305,6 +308,7 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
 #undef UP
   d.ncurves = L;
   d.ndata = N;
+  d.ntiles = (int)tcv.size();
   d.nfac = F;
   d.lmax = p->lmax;
   d.mmax = p->mmax;
@@ -353,8 +357,7 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   c->n_iter_max = d.n_iter_max;
   c->escl = p->escl;
   c->max_lp = maxlp;
-  c->jrows = ((maxlp + 15) / 16) * 16;
-  c->stride = (long long)ps_fit_scratch_stride(d, c->jrows);
+  c->stride = (long long)ps_slab(d).stride;
   const size_t lds_cap = 160 * 1024;
   d.ylds = ps_fit_lds_bytes(d, true) <= lds_cap ? 1 : 0;
   c->lds = ps_fit_lds_bytes(d, d.ylds != 0);
@@ -568,10 +571,10 @@ int ps_eval(ps_ctx* c, const double* a, double conw_r2_rows, double* alpha, doub
     if (area) std::memcpy(area, ar.data(), ar.size() * sizeof(double));
   }
   if (jrows) {
-    PS_HIP(c, hipMalloc(&dj, (size_t)c->ndata * (ma + 1) * sizeof(double)));
+    PS_HIP(c, hipMalloc(&dj, (size_t)(c->ndata + 3) * (ma + 1) * sizeof(double)));
     PS_HIP(c, ps_launch_eval(A, c->stream, da, dw, dar, dj));
     PS_HIP(c, hipStreamSynchronize(c->stream));
-    PS_HIP(c, hipMemcpy(jrows, dj, (size_t)c->ndata * (ma + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    PS_HIP(c, hipMemcpy(jrows, dj, (size_t)(c->ndata + 3) * (ma + 1) * sizeof(double), hipMemcpyDeviceToHost));
     hipFree(dj);
   }
   hipFree(da);

@@ -194,20 +194,32 @@ __device__ __forceinline__ void point_geometry(const PsDevProblem& P, const Inv&
 
 // LDS layout of one workgroup (dynamic shared memory).
 struct Lds {
-  double* work;    // [K*K] normal matrix target / Gauss-Jordan matrix
+  double* work;    // [K*K] normal matrix / augmented Gauss-Jordan matrix
   double* nrm;     // [F][4] facet normal x, y, z and the current facet area
   double* ytab;    // [F][16][nbs] shape basis (LDS copy, or the global table)
   double* cgg;     // [ma]
   double* atry;    // [ma]
-  double* dvec;    // [qs] column sums of the current curve
   double* rowbuf;  // [NW][16][16] per-point non-shape values
-  double* x;       // [K] right-hand side / solution
-  double* dum;     // [K] elimination multipliers
-  double* red;     // [2*NW] reduction scratch
+  double* red;     // [8] reduction scratch
   double* inv;     // [16] pair invariants of the evaluated parameters
-  int* ipiv;       // [K]
-  int* ired;       // [2*NW]
 };
+
+// Pointers into one workgroup's scratch slab.
+struct Slab {
+  double *jbuf, *ybuf, *tsum, *dsum, *part, *best;
+};
+
+__device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
+  const PsSlab o = ps_slab(P);
+  Slab s;
+  s.jbuf = base + o.jbuf;
+  s.ybuf = base + o.ybuf;
+  s.tsum = base + o.tsum;
+  s.dsum = base + o.dsum;
+  s.part = base + o.part;
+  s.best = base + o.best;
+  return s;
+}
 
 __device__ Lds carve_lds(double* base, const PsDevProblem& P, bool ylds) {
   Lds L;
@@ -227,21 +239,11 @@ __device__ Lds carve_lds(double* base, const PsDevProblem& P, bool ylds) {
   p += P.ma;
   L.atry = p;
   p += P.ma;
-  L.dvec = p;
-  p += P.qs;
   L.rowbuf = p;
   p += NW * 256;
-  L.x = p;
-  p += K;
-  L.dum = p;
-  p += K;
   L.red = p;
-  p += 2 * NW;
+  p += 8;
   L.inv = p;
-  p += 16;
-  L.ipiv = reinterpret_cast<int*>(p);
-  p += (K + 1) / 2 + 1;
-  L.ired = reinterpret_cast<int*>(p);
   return L;
 }
 
@@ -293,7 +295,7 @@ __device__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) 
 // Phase 1 for one 16-point tile of one curve, executed by one wave.
 template <int NB, int NBS>
 __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int lp, int t, double* jbuf,
-                                double* ybuf, double* rowbuf) {
+                                double* ybuf, double* tsum, double* rowbuf) {
   const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
   const Inv v = load_inv(L.inv);
   const int F = P.nfac, QS = P.qs, C = P.ncoef;
@@ -403,198 +405,252 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int
     rb[9] = sc;
   }
   wave_sync();
+  // rows out (global point index p0 + prow) and the tile's column sums:
+  // lane group sl sums the points 4r + sl in order, the groups combine as
+  // (G0 + G1) + (G2 + G3); padding rows add +0.0
+  double tsq[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) tsq[b] = 0.0;
+  double tsy = 0.0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int ptl = 4 * r + sl;
     const int prow = t * 16 + ptl;
-    if (prow < lp) {
-      const double* rb = rowbuf + ptl * 16;
-      double* jr = jbuf + (size_t)prow * QS;
+    const bool ok = prow < lp;
+    const double* rb = rowbuf + ptl * 16;
+    double* jr = jbuf + (size_t)(p0 + (ok ? prow : 0)) * QS;
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int q = 16 * b + ps;
-        double val = 0.0;
-        if (q <= C - 2) {
-          if (b < NBS) val = rb[9] * gat[b < NBS ? b : 0][r];
-        } else if (q <= C + 6) {
-          val = rb[q - (C - 1)];
-        }
-        jr[q] = val;
+    for (int b = 0; b < NB; ++b) {
+      const int q = 16 * b + ps;
+      double val = 0.0;
+      if (q <= C - 2) {
+        if (b < NBS) val = rb[9] * gat[b < NBS ? b : 0][r];
+      } else if (q <= C + 6) {
+        val = rb[q - (C - 1)];
       }
-      if (ps == 0) ybuf[prow] = rb[8];
+      if (ok) jr[q] = val;
+      tsq[b] = tsq[b] + (ok ? val : 0.0);
     }
+    const double yv = rb[8];
+    if (ok && ps == 0) ybuf[p0 + prow] = yv;
+    tsy = tsy + (ok ? yv : 0.0);
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) tsq[b] = xsum16_32(tsq[b]);
+  tsy = xsum16_32(tsy);
+  if (sl == 0) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) tsum[16 * b + ps] = tsq[b];
+    if (ps == 0) tsum[QS] = tsy;
   }
   wave_sync();
   return nvis;
 }
 
-// Phase 2 for one tile: X = [J' | r], W = s2w; the augmented normal matrix
-// gains X^T (W X).  Every wave walks every tile and owns the 16x16 block
-// pairs (ib >= jb) whose enumeration index is = wave (mod NW).
-// `conv` selects the three convexity rows (no renormalisation).
+// ---------------------------------------------------------------------------
+// Phase 2: the augmented normal matrix [J'|r]^T W [J'|r] on FP64 MFMA.
+// The waves form P2<NB>::GT point chains; chain g accumulates the tiles
+// T = g (mod GT) (data tiles in curve order, then the convexity tile) and,
+// for NB > 4, its two waves split the 16x16 block pairs (ib >= jb).  Every
+// element of a chain is one fused multiply-add chain over its points in
+// order; the chains' partial matrices go to the slab and are added as
+// (C0 + C1) + (C2 + C3) (GT = 4) or C0 + C1 (GT = 2).
+// ---------------------------------------------------------------------------
 template <int NB>
-__device__ void phase2_tile(const PsDevProblem& P, int p0, int lp, int t, const double* jbuf, const double* ybuf,
-                            const double* dvec, double rave, bool conv, double conv_w, v4d* acc) {
-  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
-  const int mf = P.mfit, QS = P.qs, C = P.ncoef;
+struct P2 {
+  static constexpr int NP = NB * (NB + 1) / 2;  // lower block pairs
+  static constexpr int GT = NB <= 4 ? 4 : 2;    // point chains
+  static constexpr int GB = NW / GT;            // waves per chain
+  static constexpr int NPW = (NP + GB - 1) / GB;
+};
+static_assert(P2<4>::GT == 4 && P2<5>::GT == 2, "chain count must match ps_tile_chains");
+
+// Raw inputs of one tile for one wave: lane (ps, sl) holds column 16b + ps
+// of the points 4r + sl of the tile.
+template <int NB>
+struct P2Tile {
+  double J[4][NB];
+  double y[4], sig[4], obs[4], s2[4];
+  double dv[NB];  // column sums of the tile's curve
+  double rave;    // 1 / brightness sum of the curve
+  int lp, t0;     // points of the curve, first point of the tile in it
+  bool conv;
+
+  __device__ __forceinline__ void load(const PsDevProblem& P, const Slab& S, int T) {
+    const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
+    const int QS = P.qs;
+    conv = T >= P.ntiles;
+    // the convexity tile reads curve 0's sums and point 0's observations
+    // (discarded): every load stays in bounds without branches
+    const int c = conv ? 0 : P.tile_cv[T];
+    const int cp0 = P.curve_off[c];
+    const int p0 = conv ? P.ndata : cp0;
+    lp = conv ? 3 : P.curve_off[c + 1] - cp0;
+    t0 = conv ? 0 : 16 * (T - P.tile_off[c]);
+    const double* ds = S.dsum + (size_t)c * (QS + 1);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int pin = t * 16 + 4 * r + sl;
-    const bool valid = pin < lp;
-    double coef = 0.0, coef1 = 0.0, res = 0.0, s2 = 0.0;
-    if (valid) {
-      const double y = ybuf[pin];
-      if (conv) {
-        res = 0.0 - y;
-        s2 = conv_w * 1.0;
-      } else {
-        const int p = p0 + pin;
-        coef = (P.sig[p] * (double)lp) * rave;
-        coef1 = y * rave;
-        const double ym = coef * y;
-        res = P.bright[p] - ym;
-        s2 = P.s2w[p];
-      }
+    for (int b = 0; b < NB; ++b) dv[b] = ds[16 * b + ps];
+    rave = ds[QS];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int pin = t0 + 4 * r + sl;
+      const int pt = p0 + (pin < lp ? pin : 0);
+      const int po = conv ? 0 : pt;
+      const double* jr = S.jbuf + (size_t)pt * QS;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) J[r][b] = jr[16 * b + ps];
+      y[r] = S.ybuf[pt];
+      sig[r] = P.sig[po];
+      obs[r] = P.bright[po];
+      s2[r] = P.s2w[po];
     }
-    const double* jr = jbuf + (size_t)(valid ? pin : 0) * QS;
-    double X[NB], XW[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int q = 16 * b + ps;
-      double xv = 0.0;
-      if (valid) {
-        if (q < mf) {
-          if (conv)
-            xv = (q + 2 <= C) ? jr[q] : 0.0;
-          else
-            xv = coef * fma(-coef1, dvec[q], jr[q]);
-        } else if (q == mf) {
-          xv = res;
-        }
-      }
-      X[b] = xv;
-      XW[b] = s2 * xv;
-    }
-    int idx = 0;
-#pragma unroll
-    for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-      for (int jb = 0; jb <= ib; ++jb) {
-        if (idx % NW == w) acc[idx / NW] = mfma(X[ib], XW[jb], acc[idx / NW]);
-        ++idx;
-      }
   }
-}
 
-// The augmented normal matrix lives in L.work (lower triangle, stride K)
-// between phase-2 passes; each wave moves its own 16x16 block pairs between
-// LDS and MFMA accumulators, so every element stays one fused multiply-add
-// chain over the points while the accumulators are not live in phase 1.
+  template <int NPW, int GB>
+  __device__ __forceinline__ void accumulate(int mf, int C, double conw_w, int gb, v4d* acc) const {
+    const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool ok = t0 + 4 * r + sl < lp;
+      double coef = 0.0, coef1 = 0.0, res = 0.0, s2v = 0.0;
+      if (ok) {
+        if (conv) {
+          res = 0.0 - y[r];
+          s2v = conw_w * 1.0;
+        } else {
+          coef = (sig[r] * (double)lp) * rave;
+          coef1 = y[r] * rave;
+          const double ym = coef * y[r];
+          res = obs[r] - ym;
+          s2v = s2[r];
+        }
+      }
+      double X[NB], XW[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int q = 16 * b + ps;
+        double xv = 0.0;
+        if (ok) {
+          if (q < mf) {
+            if (conv)
+              xv = (q + 2 <= C) ? J[r][b] : 0.0;
+            else
+              xv = coef * fma(-coef1, dv[b], J[r][b]);
+          } else if (q == mf) {
+            xv = res;
+          }
+        }
+        X[b] = xv;
+        XW[b] = s2v * xv;
+      }
+      int idx = 0;
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int jb = 0; jb <= ib; ++jb) {
+          if (idx % GB == gb) acc[idx / GB] = mfma(X[ib], XW[jb], acc[idx / GB]);
+          ++idx;
+        }
+    }
+  }
+};
+
+// One wave's share of phase 2: all loads of a tile are issued before its
+// products.  Writes the chain's partial matrix blocks.
 template <int NB>
-__device__ __forceinline__ void blocks_load(const double* work, int K, v4d* acc) {
+__device__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w) {
+  using SH = P2<NB>;
   const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
+  const int gt = w / SH::GB, gb = w % SH::GB;
+  const int mf = P.mfit, C = P.ncoef;
+  v4d acc[SH::NPW];
+#pragma unroll
+  for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int ntot = P.ntiles + 1;  // data tiles + the convexity tile
+  for (int T = gt; T < ntot; T += SH::GT) {
+    P2Tile<NB> cur;
+    cur.load(P, S, T);
+    cur.template accumulate<SH::NPW, SH::GB>(mf, C, conw_w, gb, acc);
+  }
+  double* pt = S.part + (size_t)gt * 256 * NB * NB;
+  const int ld = 16 * NB;
   int idx = 0;
 #pragma unroll
   for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
     for (int jb = 0; jb <= ib; ++jb) {
-      if (idx % NW == w) {
+      if (idx % SH::GB == gb) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int ai = 16 * ib + 4 * r + sl, bj = 16 * jb + ps;
-          acc[idx / NW][r] = (ai < K && bj <= ai) ? work[ai * K + bj] : 0.0;
-        }
+        for (int r = 0; r < 4; ++r) pt[(size_t)(16 * ib + 4 * r + sl) * ld + 16 * jb + ps] = acc[idx / SH::GB][r];
       }
       ++idx;
     }
 }
 
-template <int NB>
-__device__ __forceinline__ void blocks_store(double* work, int K, const v4d* acc) {
-  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
-  int idx = 0;
-#pragma unroll
-  for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-    for (int jb = 0; jb <= ib; ++jb) {
-      if (idx % NW == w) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int ai = 16 * ib + 4 * r + sl, bj = 16 * jb + ps;
-          if (ai < K && bj <= ai) work[ai * K + bj] = acc[idx / NW][r];
-        }
-      }
-      ++idx;
-    }
-}
-
-// One mrqcof evaluation at a[] (LDS) into L.work (K*K, symmetric alpha,
-// beta in row mfit, chi^2 at [mfit][mfit]); facet areas of a in L.nrm[4f+3].
+// One mrqcof evaluation at a[] (LDS) into L.work (K*K: symmetric alpha,
+// beta in row mfit, chi^2 at [mfit][mfit]); facet areas of a in L.nrm[4f+3];
+// raw Jacobian rows of every point (and the convexity rows) in the slab.
 template <int NB, int NBS>
-__device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, double* jbuf,
-                         double* ybuf, unsigned long long* vis_count) {
-  constexpr int NP = NB * (NB + 1) / 2;
-  constexpr int NPW = (NP + NW - 1) / NW;
+__device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, const Slab& S,
+                         unsigned long long* vis_count) {
+  using SH = P2<NB>;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac;
+  const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac, N = P.ndata;
   PS_TSTART();
   curv_block(P, a, L.nrm);
   if (tid == 0) store_inv(make_inv(P, a), L.inv);
-  for (int e = tid; e < K * K; e += WG) L.work[e] = 0.0;
   __syncthreads();
   PS_TMARK(0);
+  // convexity pseudo-curve (Start.cu:2539-2612), rows N..N+2:
+  // y_j = sum Area n_j, J_j[c] = sum (Area n_j) Y_c
+  for (int idx = tid; idx < 3 * QS; idx += WG) {
+    const int j = idx / QS, q = idx - j * QS;
+    double s = 0.0;
+    if (q <= C - 2)
+      for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3] * L.nrm[4 * f + j], ybasis(P, L.ytab, f, q), s);
+    S.jbuf[(size_t)(N + j) * QS + q] = s;
+    if (q == QS - 1) {
+      double ys = 0.0;
+      for (int f = 0; f < F; ++f) ys = ys + L.nrm[4 * f + 3] * L.nrm[4 * f + j];
+      S.ybuf[N + j] = ys;
+    }
+  }
+  // phase 1 over every tile of every curve
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
-  for (int c = 0; c < P.ncurves; ++c) {
+  for (int T = w; T < P.ntiles; T += NW) {
+    const int c = P.tile_cv[T];
     const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
-    const int ntile = (lp + 15) / 16;
-    for (int t = w; t < ntile; t += NW) nvis += phase1_tile<NB, NBS>(P, L, p0, lp, t, jbuf, ybuf, rowbuf);
-    PS_TMARK(1);
-    __syncthreads();
-    PS_TMARK(2);
-    // curve sums, sequential over points (column q = parameter q+2)
-    for (int q = tid; q <= mf; q += WG) {
-      double s = 0.0;
-      if (q < mf) {
-        for (int j = 0; j < lp; ++j) s = s + jbuf[(size_t)j * QS + q];
-      } else {
-        for (int j = 0; j < lp; ++j) s = s + ybuf[j];
-      }
-      L.dvec[q] = s;
-    }
-    __syncthreads();
-    PS_TMARK(3);
-    {
-      const double rave = 1.0 / L.dvec[mf];
-      v4d acc[NPW];
-      blocks_load<NB>(L.work, K, acc);
-      for (int t = 0; t < ntile; ++t) phase2_tile<NB>(P, p0, lp, t, jbuf, ybuf, L.dvec, rave, false, 0.0, acc);
-      blocks_store<NB>(L.work, K, acc);
-    }
-    PS_TMARK(4);
-    __syncthreads();
-    PS_TMARK(5);
+    nvis += phase1_tile<NB, NBS>(P, L, p0, lp, T - P.tile_off[c], S.jbuf, S.ybuf, S.tsum + (size_t)T * (QS + 1),
+                                 rowbuf);
   }
-  if (vis_count && lane == 0) atomicAdd(vis_count, nvis);
-  // convexity pseudo-curve (Start.cu:2539-2612): y_j = sum Area n_j,
-  // J_j[c] = sum (Area n_j) Y_c
-  for (int idx = tid; idx < 3 * C; idx += WG) {
-    const int j = idx / C, c = idx - j * C;
+  PS_TMARK(1);
+  __syncthreads();
+  PS_TMARK(2);
+  // curve sums: the tile sums of each curve in order (column qs: brightness,
+  // stored as its reciprocal)
+  for (int e = tid; e < P.ncurves * K; e += WG) {
+    const int c = e / K, q = e - c * K;
+    const int col = q < mf ? q : QS;
     double s = 0.0;
-    if (c == 0) {
-      for (int f = 0; f < F; ++f) s = s + L.nrm[4 * f + 3] * L.nrm[4 * f + j];
-      ybuf[j] = s;
-    } else {
-      for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3] * L.nrm[4 * f + j], ybasis(P, L.ytab, f, c - 1), s);
-      jbuf[(size_t)j * QS + (c - 1)] = s;
-    }
+    for (int T = P.tile_off[c]; T < P.tile_off[c + 1]; ++T) s = s + S.tsum[(size_t)T * (QS + 1) + col];
+    S.dsum[(size_t)c * (QS + 1) + col] = q < mf ? s : 1.0 / s;
   }
   __syncthreads();
+  PS_TMARK(3);
+  phase2_chain<NB>(P, S, conw_r2);
+  PS_TMARK(4);
+  __syncthreads();
+  PS_TMARK(5);
   {
-    v4d acc[NPW];
-    blocks_load<NB>(L.work, K, acc);
-    phase2_tile<NB>(P, 0, 3, 0, jbuf, ybuf, L.dvec, 0.0, true, conw_r2, acc);
-    blocks_store<NB>(L.work, K, acc);
+    const int ld = 16 * NB;
+    const size_t ps_ = (size_t)256 * NB * NB;
+    for (int e = tid; e < K * K; e += WG) {
+      const int i = e / K, j = e - i * K;
+      if (j > i) continue;
+      const double* c0 = S.part + (size_t)i * ld + j;
+      L.work[e] = SH::GT == 4 ? (c0[0] + c0[ps_]) + (c0[2 * ps_] + c0[3 * ps_]) : c0[0] + c0[ps_];
+    }
   }
   __syncthreads();
   for (int e = tid; e < mf * mf; e += WG) {
@@ -602,109 +658,101 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
     if (j > i) L.work[i * K + j] = L.work[j * K + i];
   }
   __syncthreads();
+  if (vis_count && lane == 0) atomicAdd(vis_count, nvis);
   PS_TMARK(6);
 }
 
-// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) on L.work (stride K,
-// mf x mf) with right-hand side L.x.  NR gaussj pivot rule: the last
-// occurrence of the largest |entry| in row-major order.  Returns the
-// reference error code (0, 1 = ipiv > 1, 2 = zero pivot / nothing to pick).
-__device__ int gaussj_block(const Lds& L, int mf, int K) {
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  for (int j = tid; j < mf; j += WG) L.ipiv[j] = 0;
-  __syncthreads();
-  for (int i = 0; i < mf; ++i) {
-    // row scan: thread j scans row j
-    double big = 0.0;
-    int brow = -1, bcol = -1, e1 = 0;
-    for (int j = tid; j < mf; j += WG) {
-      if (L.ipiv[j] == 1) continue;
-      for (int k = 0; k < mf; ++k) {
-        const int ik = L.ipiv[k];
-        if (ik == 0) {
-          const double tv = fabs(L.work[j * K + k]);
-          if (tv >= big) {
-            big = tv;
-            brow = j;
-            bcol = k;
-          }
-        } else if (ik > 1) {
-          e1 = 1;
-        }
-      }
+// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) by one wave on the
+// augmented matrix [A | b] in LDS (rows 0..mf-1, stride K = mf + 1, b in
+// column mf).  Lane l owns columns l and l + 64.  NR gaussj pivot rule: the
+// largest |entry| among unpivoted rows and columns, the last occurrence in
+// row-major order on ties; the search for the next pivot runs inside the
+// elimination.  Returns 0, or 2 for a zero pivot (or nothing selectable);
+// the solution is left in column mf.
+__device__ int gaussj_wave(double* a, int mf, int K) {
+  const int lane = threadIdx.x & 63;
+  const int c0 = lane, c1 = lane + 64;
+  const bool h0 = c0 < K, h1 = c1 < K;
+  unsigned long long u0 = 0, u1 = 0;  // pivoted rows / columns
+  auto used = [&](int k) -> bool { return k < 64 ? (u0 >> k) & 1ull : (u1 >> (k - 64)) & 1ull; };
+  double big = 0.0;
+  int brow = -1, bcol = -1;
+  auto consider = [&](double v, int row, int col) {
+    const double t = fabs(v);
+    if (t >= big) {
+      big = t;
+      brow = row;
+      bcol = col;
     }
-    // block argmax: larger |value|, ties -> larger row
+  };
+  for (int j = 0; j < mf; ++j) {
+    if (c0 < mf) consider(a[j * K + c0], j, c0);
+    if (c1 < mf) consider(a[j * K + c1], j, c1);
+  }
+  for (int i = 0; i < mf; ++i) {
+    // wave argmax of (|value|, row, column)
+#pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const double ob = __shfl_xor(big, off);
       const int orow = __shfl_xor(brow, off), ocol = __shfl_xor(bcol, off);
-      const bool take = orow >= 0 && (brow < 0 || ob > big || (ob == big && orow > brow));
+      const bool take =
+          orow >= 0 && (brow < 0 || ob > big || (ob == big && (orow > brow || (orow == brow && ocol > bcol))));
       if (take) {
         big = ob;
         brow = orow;
         bcol = ocol;
       }
     }
-    const int e1w = __any(e1);
-    if (lane == 0) {
-      L.red[w] = big;
-      L.ired[w] = brow;
-      L.ired[NW + w] = bcol;
-      L.red[NW + w] = e1w;
-    }
-    __syncthreads();
-    int err = 0, irow = -1, icol = -1;
-    {
-      double b = 0.0;
-      for (int ww = 0; ww < NW; ++ww) {
-        if (L.red[NW + ww] != 0.0) err = 1;
-        const int orow = L.ired[ww];
-        const double ob = L.red[ww];
-        if (orow >= 0 && (irow < 0 || ob > b || (ob == b && orow > irow))) {
-          b = ob;
-          irow = orow;
-          icol = L.ired[NW + ww];
-        }
-      }
-    }
-    if (err) return 1;
+    const int irow = brow, icol = bcol;
     if (icol < 0) return 2;
-    __syncthreads();
-    if (tid == 0) L.ipiv[icol] += 1;
-    if (irow != icol) {
-      for (int l = tid; l < mf; l += WG) {
-        const double tmp = L.work[irow * K + l];
-        L.work[irow * K + l] = L.work[icol * K + l];
-        L.work[icol * K + l] = tmp;
-      }
-      if (tid == 0) {
-        const double tmp = L.x[irow];
-        L.x[irow] = L.x[icol];
-        L.x[icol] = tmp;
+    if (icol < 64)
+      u0 |= 1ull << icol;
+    else
+      u1 |= 1ull << (icol - 64);
+    // swap rows irow and icol
+    double pv0 = 0.0, pv1 = 0.0;
+    if (h0) {
+      pv0 = a[irow * K + c0];
+      if (irow != icol) {
+        a[irow * K + c0] = a[icol * K + c0];
+        a[icol * K + c0] = pv0;
       }
     }
-    __syncthreads();
-    const double piv = L.work[icol * K + icol];
+    if (h1) {
+      pv1 = a[irow * K + c1];
+      if (irow != icol) {
+        a[irow * K + c1] = a[icol * K + c1];
+        a[icol * K + c1] = pv1;
+      }
+    }
+    const double piv = __shfl(icol < 64 ? pv0 : pv1, icol & 63);
     if (piv == 0.0) return 2;
     const double pivinv = 1.0 / piv;
-    __syncthreads();
-    if (tid == 0) {
-      L.work[icol * K + icol] = 1.0;
-      L.x[icol] *= pivinv;
-    }
-    __syncthreads();
-    for (int l = tid; l < mf; l += WG) L.work[icol * K + l] *= pivinv;
-    for (int l = tid; l < mf; l += WG) L.dum[l] = L.work[l * K + icol];
-    __syncthreads();
-    for (int e = tid; e < mf * mf; e += WG) {
-      const int ll = e / mf, c = e - ll * mf;
+    const double pn0 = (c0 == icol ? 1.0 : pv0) * pivinv;
+    const double pn1 = (c1 == icol ? 1.0 : pv1) * pivinv;
+    if (h0) a[icol * K + c0] = pn0;
+    if (h1) a[icol * K + c1] = pn1;
+    wave_sync();
+    big = 0.0;
+    brow = bcol = -1;
+    for (int ll = 0; ll < mf; ++ll) {
       if (ll == icol) continue;
-      const double dm = L.dum[ll];
-      const double base = (c == icol) ? 0.0 : L.work[ll * K + c];
-      L.work[ll * K + c] = fma(-L.work[icol * K + c], dm, base);
+      const double dum = a[ll * K + icol];
+      const bool cand = !used(ll);
+      if (h0) {
+        const double base = (c0 == icol) ? 0.0 : a[ll * K + c0];
+        const double nv = fma(-pn0, dum, base);
+        a[ll * K + c0] = nv;
+        if (cand && c0 < mf && !used(c0)) consider(nv, ll, c0);
+      }
+      if (h1) {
+        const double base = (c1 == icol) ? 0.0 : a[ll * K + c1];
+        const double nv = fma(-pn1, dum, base);
+        a[ll * K + c1] = nv;
+        if (cand && c1 < mf && !used(c1)) consider(nv, ll, c1);
+      }
     }
-    for (int ll = tid; ll < mf; ll += WG)
-      if (ll != icol) L.x[ll] = fma(-L.x[icol], L.dum[ll], L.x[ll]);
-    __syncthreads();
+    wave_sync();
   }
   return 0;
 }
@@ -712,7 +760,7 @@ __device__ int gaussj_block(const Lds& L, int mf, int K) {
 struct PairState {
   double alamda, chisq, ochisq, iter_diff, dev_old, dev_new, chck;
   unsigned long long nvis;  // visible (point, facet) pairs over all evaluations
-  int niter, flags;
+  int niter, gj_err;
 };
 
 }  // namespace
@@ -730,9 +778,8 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
   stage_tables(P, L, YLDS);
   const int tid = threadIdx.x;
   const int mf = P.mfit, K = mf + 1, C = P.ncoef, ma = P.ma, F = P.nfac;
-  double* jbuf = A.scratch + (size_t)blockIdx.x * A.scratch_stride;
-  double* ybuf = jbuf + (size_t)A.jrows * P.qs;
-  double* best = ybuf + A.jrows;  // [K*K] accepted alpha | beta | chi^2
+  const Slab SL = make_slab(P, A.scratch + (size_t)blockIdx.x * A.scratch_stride);
+  double* best = SL.best;  // [K*K] accepted alpha | beta | chi^2
   for (;;) {
     if (tid == 0) item_s = atomicAdd(A.counter, 1);
     __syncthreads();
@@ -779,23 +826,32 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
       for (int pass = is_alambda ? 0 : 1; pass < 2; ++pass) {
         if (pass == 1) {
           PS_TSTART();
+          // [alpha (diagonal * (1 + lambda)) | beta]
           const double ccc = 1 + S.alamda;
-          for (int e = tid; e < mf * mf; e += WG) {
-            const int i = e / mf, j = e - i * mf;
-            double vv = best[i * K + j];
-            if (i == j) vv *= ccc;
-            L.work[i * K + j] = vv;
+          for (int e = tid; e < mf * K; e += WG) {
+            const int i = e / K, j = e - i * K;
+            double vv;
+            if (j < mf) {
+              vv = best[i * K + j];
+              if (i == j) vv *= ccc;
+            } else {
+              vv = best[mf * K + i];
+            }
+            L.work[e] = vv;
           }
-          for (int j = tid; j < mf; j += WG) L.x[j] = best[mf * K + j];
           __syncthreads();
-          const int err = gaussj_block(L, mf, K);
+          if ((tid >> 6) == 0) {
+            const int e = gaussj_wave(L.work, mf, K);
+            if (tid == 0) S.gj_err = e;
+          }
           __syncthreads();
+          const int err = S.gj_err;
           if (err == 0 || err == 2)
-            for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.x[q];
+            for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.work[q * K + mf];
           __syncthreads();
           PS_TMARK(7);
         }
-        evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, A.conw_r2_rows, jbuf, ybuf, &S.nvis);
+        evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, A.conw_r2_rows, SL, &S.nvis);
         if (pass == 0) {
           for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
           if (tid == 0) S.ochisq = L.work[mf * K + mf];
@@ -894,66 +950,56 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
   const Lds L = carve_lds(lds_dyn, P, YLDS);
   stage_tables(P, L, YLDS);
   const int tid = threadIdx.x, K = P.mfit + 1;
-  double* jbuf = A.scratch;
-  double* ybuf = jbuf + (size_t)A.jrows * P.qs;
+  const Slab SL = make_slab(P, A.scratch);
   for (int i = tid; i < P.ma; i += WG) L.cgg[i] = a[i];
   __syncthreads();
-  if (out_j == nullptr) {
-    evaluate<NB, NBS>(P, L, L.cgg, A.conw_r2_rows, jbuf, ybuf, nullptr);
-  } else {
-    // raw Jacobian rows of every data point: phase 1 only, curve by curve
-    curv_block(P, L.cgg, L.nrm);
-    if (tid == 0) store_inv(make_inv(P, L.cgg), L.inv);
-    __syncthreads();
-    const int w = tid >> 6;
-    for (int c = 0; c < P.ncurves; ++c) {
-      const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
-      for (int t = w; t < (lp + 15) / 16; t += NW)
-        phase1_tile<NB, NBS>(P, L, p0, lp, t, jbuf, ybuf, L.rowbuf + w * 256);
-      __syncthreads();
-      for (int e = tid; e < lp * (P.ma + 1); e += WG) {
-        const int j = e / (P.ma + 1), col = e - j * (P.ma + 1);
-        double val;
-        if (col == 0)
-          val = ybuf[j];
-        else if (col == 1)
-          val = 0.0;
-        else
-          val = (col - 2 < P.qs) ? jbuf[(size_t)j * P.qs + (col - 2)] : 0.0;
-        out_j[(size_t)(p0 + j) * (P.ma + 1) + col] = val;
-      }
-      __syncthreads();
-    }
-  }
+  evaluate<NB, NBS>(P, L, L.cgg, A.conw_r2_rows, SL, nullptr);
   for (int e = tid; e < K * K; e += WG) out_work[e] = L.work[e];
   for (int f = tid; f < P.nfac; f += WG) out_area[f] = L.nrm[4 * f + 3];
+  if (out_j != nullptr) {
+    // raw rows in the reference's layout: column 0 = model brightness,
+    // column c = d/d(parameter c) (column 1, the size scale, is not computed)
+    const int W = P.ma + 1;
+    for (int e = tid; e < (P.ndata + 3) * W; e += WG) {
+      const int j = e / W, col = e - j * W;
+      double val;
+      if (col == 0)
+        val = SL.ybuf[j];
+      else if (col == 1)
+        val = 0.0;
+      else
+        val = (col - 2 < P.qs) ? SL.jbuf[(size_t)j * P.qs + (col - 2)] : 0.0;
+      out_j[e] = val;
+    }
+  }
 }
 
+// Gauss-Jordan of one damped system, by wave 0 (the parity tests' entry).
 __global__ __launch_bounds__(WG) void ps_gaussj_kernel(int mf, const double* alpha, const double* beta, double lambda,
                                                        double* x, int* err) {
   extern __shared__ double lds_dyn[];
   const int K = mf + 1;
-  Lds L;
-  L.work = lds_dyn;
-  L.x = L.work + K * K;
-  L.dum = L.x + K;
-  L.red = L.dum + K;
-  L.ipiv = reinterpret_cast<int*>(L.red + 2 * NW);
-  L.ired = L.ipiv + K + 1;
+  double* work = lds_dyn;
   const int tid = threadIdx.x;
   const double ccc = 1 + lambda;
-  for (int e = tid; e < mf * mf; e += WG) {
-    const int i = e / mf, j = e - i * mf;
-    double v = alpha[e];
-    if (i == j) v *= ccc;
-    L.work[i * K + j] = v;
+  for (int e = tid; e < mf * K; e += WG) {
+    const int i = e / K, j = e - i * K;
+    double v;
+    if (j < mf) {
+      v = alpha[i * mf + j];
+      if (i == j) v *= ccc;
+    } else {
+      v = beta[i];
+    }
+    work[e] = v;
   }
-  for (int j = tid; j < mf; j += WG) L.x[j] = beta[j];
   __syncthreads();
-  const int e = gaussj_block(L, mf, K);
-  __syncthreads();
-  for (int j = tid; j < mf; j += WG) x[j] = L.x[j];
-  if (tid == 0) *err = e;
+  if (tid < 64) {
+    const int e = gaussj_wave(work, mf, K);
+    wave_sync();
+    for (int j = tid; j < mf; j += 64) x[j] = work[j * K + mf];
+    if (tid == 0) *err = e;
+  }
 }
 
 __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2) {
@@ -972,15 +1018,9 @@ __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2
 // ---------------------------------------------------------------------------
 size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds) {
   const int K = P.mfit + 1;
-  size_t d = (size_t)K * K + 4 * P.nfac + 2 * P.ma + P.qs + NW * 256 + 2 * K + 2 * NW + 16 + (K + 1) / 2 + 1 + 2 * NW;
+  size_t d = (size_t)K * K + 4 * P.nfac + 2 * P.ma + NW * 256 + 8 + 16;
   if (ylds) d += (size_t)P.nfac * 16 * P.nbs;
   return d * sizeof(double) + 64;
-}
-
-size_t ps_fit_scratch_stride(const PsDevProblem& P, int jrows) {
-  const int K = P.mfit + 1;
-  size_t s = (size_t)jrows * P.qs + jrows + (size_t)K * K;
-  return (s + 31) & ~(size_t)31;
 }
 
 // (NB, NBS) instantiations: NB = 16-column blocks of a Jacobian row, NBS =
@@ -1041,7 +1081,7 @@ hipError_t ps_phase_cycles(unsigned long long* out, bool reset) {
 hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda, double* x, int* err,
                             hipStream_t s) {
   const int K = mf + 1;
-  const size_t lds = ((size_t)K * K + 2 * K + 2 * NW + K + 2 + 2 * NW) * sizeof(double);
+  const size_t lds = (size_t)K * K * sizeof(double);
   hipLaunchKernelGGL(ps_gaussj_kernel, dim3(1), dim3(WG), lds, s, mf, alpha, beta, lambda, x, err);
   return hipGetLastError();
 }

@@ -16,6 +16,10 @@ struct PsDevProblem {
   const double *bright, *sig, *s2w;                  // brightness, curve mean, sigr2*weight
   const int* curve_off;                              // [ncurves+1]
   int ncurves, ndata;
+  // 16-point tiles of the curves (tiles never straddle curves)
+  const int* tile_off;  // [ncurves+1] first tile of each curve
+  const int* tile_cv;   // [ntiles] curve of each tile
+  int ntiles;
   // facets and basis
   int nfac, lmax, mmax, ncoef, ma, mfit;
   int qs;   // columns of a Jacobian row (16 * nb)
@@ -44,14 +48,50 @@ struct PsFitArgs {
   const int* item_m;    // [n_items] pole index, 1-based
   int n_items;
   int* counter;         // work-queue head, zeroed before launch
-  double* scratch;      // per-workgroup scratch
+  double* scratch;      // per-workgroup scratch slabs (ps_slab layout)
   long long scratch_stride;
-  int jrows;            // Jacobian rows held per workgroup
   double* out;          // [n_items][PS_OUT_STRIDE]
 };
 
+// Point chains of the normal-equation product: the waves of a workgroup
+// form ps_tile_chains(nb) groups; group g accumulates the tiles T = g (mod
+// groups) and the groups' partial matrices are added at the end.
+__host__ __device__ inline int ps_tile_chains(int nb) { return nb <= 4 ? 4 : 2; }
+
+// One workgroup's scratch slab in HBM (offsets in doubles, 256-byte aligned).
+struct PsSlab {
+  size_t jbuf;   // [ndata+3][qs] raw Jacobian rows (data points, then the 3 convexity rows)
+  size_t ybuf;   // [ndata+3] model brightness
+  size_t tsum;   // [ntiles][qs+1] per-tile column sums (column qs: brightness)
+  size_t dsum;   // [ncurves][qs+1] per-curve column sums (column qs: 1 / brightness sum)
+  size_t part;   // [chains][16nb][16nb] partial augmented normal matrices
+  size_t best;   // [K][K] accepted normal matrix
+  size_t stride;
+};
+
+__host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
+  auto al = [](size_t n) { return (n + 31) & ~(size_t)31; };
+  PsSlab s;
+  size_t o = 0;
+  const size_t rows = (size_t)P.ndata + 3;
+  const size_t K = (size_t)P.mfit + 1;
+  s.jbuf = o;
+  o += al(rows * P.qs);
+  s.ybuf = o;
+  o += al(rows);
+  s.tsum = o;
+  o += al((size_t)P.ntiles * (P.qs + 1));
+  s.dsum = o;
+  o += al((size_t)P.ncurves * (P.qs + 1));
+  s.part = o;
+  o += al((size_t)ps_tile_chains(P.nb) * 256 * P.nb * P.nb);
+  s.best = o;
+  o += al(K * K);
+  s.stride = o;
+  return s;
+}
+
 size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds);
-size_t ps_fit_scratch_stride(const PsDevProblem& P, int jrows);
 hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds);
 hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s);
 hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
