@@ -47,6 +47,7 @@ struct ps_ctx {
   size_t lds = 0;
   long long stride = 0;
   int grid_cap = 0;
+  int blocks_per_cu = 0;
   // batch buffers
   double* scratch = nullptr;
   size_t scratch_elems = 0;
@@ -358,13 +359,19 @@ int ps_prepare(ps_ctx* c, const ps_problem* p) {
   c->escl = p->escl;
   c->max_lp = maxlp;
   c->stride = (long long)ps_slab(d).stride;
+  // The shape basis is read from L2 by default, which leaves room for two
+  // workgroups per CU; PS_BASIS_LDS=1 stages it in LDS instead (one
+  // workgroup per CU) when it fits.
   const size_t lds_cap = 160 * 1024;
-  d.ylds = ps_fit_lds_bytes(d, true) <= lds_cap ? 1 : 0;
+  const char* bl = std::getenv("PS_BASIS_LDS");
+  d.ylds = (bl && bl[0] == '1' && ps_fit_lds_bytes(d, true) <= lds_cap) ? 1 : 0;
   c->lds = ps_fit_lds_bytes(d, d.ylds != 0);
   if (c->lds > lds_cap) return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
   PS_HIP(c, ps_set_fit_attributes(nb, d.nbs, d.ylds != 0, c->lds));
-  const int per_cu = std::max(1, std::min(4, (int)((160 * 1024) / c->lds)));
+  const int per_cu = ps_fit_blocks_per_cu(nb, d.nbs, d.ylds != 0, c->lds);
+  if (per_cu < 1) return fail(c, PS_E_HIP, "fit kernel cannot be resident (occupancy query failed)");
   c->grid_cap = c->n_cu * per_cu;
+  c->blocks_per_cu = per_cu;
   c->prepared = true;
   return PS_OK;
 }

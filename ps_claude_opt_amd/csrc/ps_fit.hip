@@ -91,6 +91,21 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane and wave index, opaque to the optimiser: values derived from them
+// are computed where each phase needs them instead of being hoisted out of
+// the persistent pair loop (where they would stay live across every phase).
+__device__ __forceinline__ int lane_id() {
+  int l = threadIdx.x & 63;
+  asm volatile("" : "+v"(l));
+  return l;
+}
+
+__device__ __forceinline__ int tid_opaque() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 __device__ __forceinline__ double xsum16_32(double v) {
   v = v + __shfl_xor(v, 16);
   v = v + __shfl_xor(v, 32);
@@ -272,7 +287,7 @@ __device__ __forceinline__ double ybasis(const PsDevProblem& P, const double* yt
 // Facet areas of parameter vector a (Start.cu:421-528) into nrm[4f+3].
 __device__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) {
   const int F = P.nfac, Lm = P.lmax, M = P.mmax;
-  for (int f = threadIdx.x; f < F; f += WG) {
+  for (int f = tid_opaque(); f < F; f += WG) {
     double g = 0.0;
     int n = 0;
     for (int l = 0; l <= Lm; ++l) {
@@ -296,7 +311,7 @@ __device__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) 
 template <int NB, int NBS>
 __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int lp, int t, double* jbuf,
                                 double* ybuf, double* tsum, double* rowbuf) {
-  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
+  const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
   const Inv v = load_inv(L.inv);
   const int F = P.nfac, QS = P.qs, C = P.ncoef;
   const int pin = t * 16 + ps;
@@ -325,45 +340,66 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int
   for (int b = 0; b < NBS; ++b) gat[b] = v4d{0.0, 0.0, 0.0, 0.0};
   const double cl = v.cl, cls = v.cls;
   // lane facet f = 4k + sl: normal/area at nrm[4f..4f+3], basis values of
-  // shape columns 16b + ps at ytab[(16f + ps) * NBS + b]
+  // shape columns 16b + ps at ytab[(16f + ps) * NBS + b].  Software
+  // pipeline: the normals are loaded one step ahead, the basis values (LDS
+  // or L2) two steps ahead.
+  const int NK = F / 4;
   const double* __restrict__ nr = L.nrm + 4 * sl;
   const double* __restrict__ yb = L.ytab + (size_t)(16 * sl + ps) * NBS;
-  unsigned nvis = 0;
-  for (int k = 0; k < F / 4; ++k) {
-    const double* nrk = nr + 16 * k;
-    const double n0 = nrk[0], n1 = nrk[1], n2 = nrk[2], ar = nrk[3];
-    const double* ybk = yb + (size_t)k * 64 * NBS;
-    double yv[NBS];
+  double n0 = nr[0], n1 = nr[1], n2 = nr[2], ar = nr[3];
+  double ya[NBS], yn[NBS];
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) yv[b] = ybk[b];
+  for (int b = 0; b < NBS; ++b) {
+    ya[b] = yb[b];
+    yn[b] = yb[(NK > 1 ? (size_t)64 * NBS : 0) + b];
+  }
+  unsigned nvis = 0;
+  for (int k = 0; k < NK; ++k) {
+    const int k1 = k + 1 < NK ? k + 1 : k;
+    const int k2 = k + 2 < NK ? k + 2 : k;
+    const double* nrn = nr + 16 * k1;
+    const double m0 = nrn[0], m1 = nrn[1], m2 = nrn[2], mar = nrn[3];
+    double yc[NBS];
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) yc[b] = yb[(size_t)k2 * 64 * NBS + b];
     const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
     const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
     const bool vis = (mu > kTiny) && (mu0 > kTiny);
     const unsigned long long bal = __ballot(vis);
-    if (bal == 0) continue;  // whole step invisible: exact zeros
-    nvis += (unsigned)__popcll(bal);
-    double w = 0.0;
-    if (vis) {
-      const double rd = recip(mu + mu0);
-      const double mm = mu * mu0;
-      const double sc = mm * fma(cls, rd, cl);
-      w = ar * sc;
-      const double q0 = mu0 * rd, q1 = mu * rd;
-      const double dsmu = fma(cls, q0 * q0, cl * mu0);
-      const double dsmu0 = fma(cls, q1 * q1, cl * mu);
-      const double ad = ar * dsmu, ad0 = ar * dsmu0;
-      U0 = fma(ad, n0, U0);
-      U1 = fma(ad, n1, U1);
-      U2 = fma(ad, n2, U2);
-      V0 = fma(ad0, n0, V0);
-      V1 = fma(ad0, n1, V1);
-      V2 = fma(ad0, n2, V2);
-      br = br + w;
-      t4 = fma(ar, mm, t4);
-      t5 = fma(ar * mm, rd, t5);
-    }
+    if (bal != 0) {  // a whole step of invisible facets adds exact zeros
+      nvis += (unsigned)__popcll(bal);
+      double w = 0.0;
+      if (vis) {
+        const double rd = recip(mu + mu0);
+        const double mm = mu * mu0;
+        const double sc = mm * fma(cls, rd, cl);
+        w = ar * sc;
+        const double q0 = mu0 * rd, q1 = mu * rd;
+        const double dsmu = fma(cls, q0 * q0, cl * mu0);
+        const double dsmu0 = fma(cls, q1 * q1, cl * mu);
+        const double ad = ar * dsmu, ad0 = ar * dsmu0;
+        U0 = fma(ad, n0, U0);
+        U1 = fma(ad, n1, U1);
+        U2 = fma(ad, n2, U2);
+        V0 = fma(ad0, n0, V0);
+        V1 = fma(ad0, n1, V1);
+        V2 = fma(ad0, n2, V2);
+        br = br + w;
+        t4 = fma(ar, mm, t4);
+        t5 = fma(ar * mm, rd, t5);
+      }
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, yv[b], gat[b]);
+      for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, ya[b], gat[b]);
+    }
+    n0 = m0;
+    n1 = m1;
+    n2 = m2;
+    ar = mar;
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) {
+      ya[b] = yn[b];
+      yn[b] = yc[b];
+    }
   }
   br = xsum16_32(br);
   t4 = xsum16_32(t4);
@@ -465,102 +501,107 @@ struct P2 {
 };
 static_assert(P2<4>::GT == 4 && P2<5>::GT == 2, "chain count must match ps_tile_chains");
 
-// Raw inputs of one tile for one wave: lane (ps, sl) holds column 16b + ps
-// of the points 4r + sl of the tile.
+// One tile of phase 2 for one wave: lane (ps, sl) handles column 16b + ps
+// of the tile's points 4r + sl, r = 0..3.
 template <int NB>
 struct P2Tile {
-  double J[4][NB];
-  double y[4], sig[4], obs[4], s2[4];
   double dv[NB];  // column sums of the tile's curve
   double rave;    // 1 / brightness sum of the curve
-  int lp, t0;     // points of the curve, first point of the tile in it
-  bool conv;
+  int p0, lp, t0; // first point of the curve, its length, first point of the tile in it
+  bool conv;      // the convexity tile (rows ndata .. ndata+2)
+
+  struct Row {
+    double J[NB];
+    double y, sig, obs, s2;
+  };
 
   __device__ __forceinline__ void load(const PsDevProblem& P, const Slab& S, int T) {
-    const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
+    const int ps = lane_id() & 15;
     const int QS = P.qs;
     conv = T >= P.ntiles;
     // the convexity tile reads curve 0's sums and point 0's observations
-    // (discarded): every load stays in bounds without branches
+    // (discarded), so every load stays in bounds without branches
     const int c = conv ? 0 : P.tile_cv[T];
     const int cp0 = P.curve_off[c];
-    const int p0 = conv ? P.ndata : cp0;
+    p0 = conv ? P.ndata : cp0;
     lp = conv ? 3 : P.curve_off[c + 1] - cp0;
     t0 = conv ? 0 : 16 * (T - P.tile_off[c]);
     const double* ds = S.dsum + (size_t)c * (QS + 1);
 #pragma unroll
     for (int b = 0; b < NB; ++b) dv[b] = ds[16 * b + ps];
     rave = ds[QS];
+  }
+
+  __device__ __forceinline__ Row row(const PsDevProblem& P, const Slab& S, int r) const {
+    const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
+    Row R;
+    const int pin = t0 + 4 * r + sl;
+    const int pt = p0 + (pin < lp ? pin : 0);
+    const int po = conv ? 0 : pt;
+    const double* jr = S.jbuf + (size_t)pt * P.qs;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int pin = t0 + 4 * r + sl;
-      const int pt = p0 + (pin < lp ? pin : 0);
-      const int po = conv ? 0 : pt;
-      const double* jr = S.jbuf + (size_t)pt * QS;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) J[r][b] = jr[16 * b + ps];
-      y[r] = S.ybuf[pt];
-      sig[r] = P.sig[po];
-      obs[r] = P.bright[po];
-      s2[r] = P.s2w[po];
-    }
+    for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b + ps];
+    R.y = S.ybuf[pt];
+    R.sig = P.sig[po];
+    R.obs = P.bright[po];
+    R.s2 = P.s2w[po];
+    return R;
   }
 
   template <int NPW, int GB>
-  __device__ __forceinline__ void accumulate(int mf, int C, double conw_w, int gb, v4d* acc) const {
-    const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const bool ok = t0 + 4 * r + sl < lp;
-      double coef = 0.0, coef1 = 0.0, res = 0.0, s2v = 0.0;
-      if (ok) {
-        if (conv) {
-          res = 0.0 - y[r];
-          s2v = conw_w * 1.0;
-        } else {
-          coef = (sig[r] * (double)lp) * rave;
-          coef1 = y[r] * rave;
-          const double ym = coef * y[r];
-          res = obs[r] - ym;
-          s2v = s2[r];
-        }
+  __device__ __forceinline__ void accumulate(const Row& R, int r, int mf, int C, double conw_w, int gb,
+                                             v4d* acc) const {
+    const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
+    const bool ok = t0 + 4 * r + sl < lp;
+    double coef = 0.0, coef1 = 0.0, res = 0.0, s2v = 0.0;
+    if (ok) {
+      if (conv) {
+        res = 0.0 - R.y;
+        s2v = conw_w * 1.0;
+      } else {
+        coef = (R.sig * (double)lp) * rave;
+        coef1 = R.y * rave;
+        const double ym = coef * R.y;
+        res = R.obs - ym;
+        s2v = R.s2;
       }
-      double X[NB], XW[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int q = 16 * b + ps;
-        double xv = 0.0;
-        if (ok) {
-          if (q < mf) {
-            if (conv)
-              xv = (q + 2 <= C) ? J[r][b] : 0.0;
-            else
-              xv = coef * fma(-coef1, dv[b], J[r][b]);
-          } else if (q == mf) {
-            xv = res;
-          }
-        }
-        X[b] = xv;
-        XW[b] = s2v * xv;
-      }
-      int idx = 0;
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int jb = 0; jb <= ib; ++jb) {
-          if (idx % GB == gb) acc[idx / GB] = mfma(X[ib], XW[jb], acc[idx / GB]);
-          ++idx;
-        }
     }
+    double X[NB], XW[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int q = 16 * b + ps;
+      double xv = 0.0;
+      if (ok) {
+        if (q < mf) {
+          if (conv)
+            xv = (q + 2 <= C) ? R.J[b] : 0.0;
+          else
+            xv = coef * fma(-coef1, dv[b], R.J[b]);
+        } else if (q == mf) {
+          xv = res;
+        }
+      }
+      X[b] = xv;
+      XW[b] = s2v * xv;
+    }
+    int idx = 0;
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int jb = 0; jb <= ib; ++jb) {
+        if (idx % GB == gb) acc[idx / GB] = mfma(X[ib], XW[jb], acc[idx / GB]);
+        ++idx;
+      }
   }
 };
 
-// One wave's share of phase 2: all loads of a tile are issued before its
-// products.  Writes the chain's partial matrix blocks.
+// One wave's share of phase 2: the next row group's loads are issued
+// before the current one's products.  Writes the chain's partial matrix.
 template <int NB>
 __device__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w) {
   using SH = P2<NB>;
-  const int lane = threadIdx.x & 63, ps = lane & 15, sl = lane >> 4, w = threadIdx.x >> 6;
+  using Row = typename P2Tile<NB>::Row;
+  const int tid = tid_opaque(), lane = tid & 63, ps = lane & 15, sl = lane >> 4, w = tid >> 6;
   const int gt = w / SH::GB, gb = w % SH::GB;
   const int mf = P.mfit, C = P.ncoef;
   v4d acc[SH::NPW];
@@ -568,9 +609,16 @@ __device__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w
   for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
   const int ntot = P.ntiles + 1;  // data tiles + the convexity tile
   for (int T = gt; T < ntot; T += SH::GT) {
-    P2Tile<NB> cur;
-    cur.load(P, S, T);
-    cur.template accumulate<SH::NPW, SH::GB>(mf, C, conw_w, gb, acc);
+    P2Tile<NB> tl;
+    tl.load(P, S, T);
+    Row cur = tl.row(P, S, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Row nxt;
+      if (r < 3) nxt = tl.row(P, S, r + 1);
+      tl.template accumulate<SH::NPW, SH::GB>(cur, r, mf, C, conw_w, gb, acc);
+      if (r < 3) cur = nxt;
+    }
   }
   double* pt = S.part + (size_t)gt * 256 * NB * NB;
   const int ld = 16 * NB;
@@ -594,7 +642,7 @@ template <int NB, int NBS>
 __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, const Slab& S,
                          unsigned long long* vis_count) {
   using SH = P2<NB>;
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int tid = tid_opaque(), w = tid >> 6, lane = tid & 63;
   const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac, N = P.ndata;
   PS_TSTART();
   curv_block(P, a, L.nrm);
@@ -662,97 +710,137 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
   PS_TMARK(6);
 }
 
-// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) by one wave on the
-// augmented matrix [A | b] in LDS (rows 0..mf-1, stride K = mf + 1, b in
-// column mf).  Lane l owns columns l and l + 64.  NR gaussj pivot rule: the
+// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) by the whole
+// workgroup on the augmented matrix [A | b] in LDS (rows 0..mf-1, stride
+// K = mf + 1, b in column mf).  Lane l of every wave owns columns l and
+// l + 64; wave w owns the rows ll = w (mod NW).  NR gaussj pivot rule: the
 // largest |entry| among unpivoted rows and columns, the last occurrence in
-// row-major order on ties; the search for the next pivot runs inside the
-// elimination.  Returns 0, or 2 for a zero pivot (or nothing selectable);
-// the solution is left in column mf.
-__device__ int gaussj_wave(double* a, int mf, int K) {
-  const int lane = threadIdx.x & 63;
+// row-major order on ties; each wave searches its rows while eliminating
+// them and the waves' candidates meet in red[0 .. 2*NW).  Two barriers per
+// pivot.  Returns 0, or 2 for a zero pivot (or nothing selectable); the
+// solution is left in column mf.  Every thread of the workgroup must call it.
+// TWO: columns beyond 64 exist (K > 64).
+template <bool TWO>
+__device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* red) {
+  const int tid = tid_opaque(), lane = tid & 63, w = tid >> 6;
   const int c0 = lane, c1 = lane + 64;
-  const bool h0 = c0 < K, h1 = c1 < K;
+  const bool h0 = c0 < K, h1 = TWO && c1 < K;
   unsigned long long u0 = 0, u1 = 0;  // pivoted rows / columns
   auto used = [&](int k) -> bool { return k < 64 ? (u0 >> k) & 1ull : (u1 >> (k - 64)) & 1ull; };
   double big = 0.0;
-  int brow = -1, bcol = -1;
-  auto consider = [&](double v, int row, int col) {
+  int bidx = -1;  // row * 128 + column
+  auto consider = [&](double v, int idx) {
     const double t = fabs(v);
     if (t >= big) {
       big = t;
-      brow = row;
-      bcol = col;
+      bidx = idx;
     }
   };
-  for (int j = 0; j < mf; ++j) {
-    if (c0 < mf) consider(a[j * K + c0], j, c0);
-    if (c1 < mf) consider(a[j * K + c1], j, c1);
+  for (int ll = w; ll < mf; ll += NW) {
+    if (c0 < mf) consider(a[ll * K + c0], ll * 128 + c0);
+    if (c1 < mf) consider(a[ll * K + c1], ll * 128 + c1);
   }
   for (int i = 0; i < mf; ++i) {
-    // wave argmax of (|value|, row, column)
+    // block argmax of (|value|, row, column)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const double ob = __shfl_xor(big, off);
-      const int orow = __shfl_xor(brow, off), ocol = __shfl_xor(bcol, off);
-      const bool take =
-          orow >= 0 && (brow < 0 || ob > big || (ob == big && (orow > brow || (orow == brow && ocol > bcol))));
-      if (take) {
+      const int oi = __shfl_xor(bidx, off);
+      if (oi >= 0 && (bidx < 0 || ob > big || (ob == big && oi > bidx))) {
         big = ob;
-        brow = orow;
-        bcol = ocol;
+        bidx = oi;
       }
     }
-    const int irow = brow, icol = bcol;
-    if (icol < 0) return 2;
+    if (lane == 0) {
+      red[w] = big;
+      red[NW + w] = (double)bidx;
+    }
+    __syncthreads();
+    big = 0.0;
+    bidx = -1;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) {
+      const double ob = red[ww];
+      const int oi = (int)red[NW + ww];
+      if (oi >= 0 && (bidx < 0 || ob > big || (ob == big && oi > bidx))) {
+        big = ob;
+        bidx = oi;
+      }
+    }
+    if (bidx < 0) return 2;
+    const int irow = bidx >> 7, icol = bidx & 127;
     if (icol < 64)
       u0 |= 1ull << icol;
     else
       u1 |= 1ull << (icol - 64);
-    // swap rows irow and icol
-    double pv0 = 0.0, pv1 = 0.0;
+    // rows irow and icol before anyone rewrites them
+    double pv0 = 0.0, pv1 = 0.0, cv0 = 0.0, cv1 = 0.0;
     if (h0) {
       pv0 = a[irow * K + c0];
-      if (irow != icol) {
-        a[irow * K + c0] = a[icol * K + c0];
-        a[icol * K + c0] = pv0;
-      }
+      cv0 = a[icol * K + c0];
     }
     if (h1) {
       pv1 = a[irow * K + c1];
-      if (irow != icol) {
-        a[irow * K + c1] = a[icol * K + c1];
-        a[icol * K + c1] = pv1;
-      }
+      cv1 = a[icol * K + c1];
     }
-    const double piv = __shfl(icol < 64 ? pv0 : pv1, icol & 63);
-    if (piv == 0.0) return 2;
+    const double piv = __shfl(icol < 64 ? pv0 : pv1, icol & 63);  // row irow, column icol
+    const double cpv = __shfl(icol < 64 ? cv0 : cv1, icol & 63);  // row icol, column icol
+    __syncthreads();
+    if (piv == 0.0) {
+      // the rows are swapped before the zero pivot is found
+      if (w == 0 && irow != icol) {
+        if (h0) {
+          a[irow * K + c0] = cv0;
+          a[icol * K + c0] = pv0;
+        }
+        if (h1) {
+          a[irow * K + c1] = cv1;
+          a[icol * K + c1] = pv1;
+        }
+      }
+      return 2;
+    }
     const double pivinv = 1.0 / piv;
     const double pn0 = (c0 == icol ? 1.0 : pv0) * pivinv;
     const double pn1 = (c1 == icol ? 1.0 : pv1) * pivinv;
-    if (h0) a[icol * K + c0] = pn0;
-    if (h1) a[icol * K + c1] = pn1;
-    wave_sync();
+    if (icol % NW == w) {
+      if (h0) a[icol * K + c0] = pn0;
+      if (h1) a[icol * K + c1] = pn1;
+    }
     big = 0.0;
-    brow = bcol = -1;
-    for (int ll = 0; ll < mf; ++ll) {
-      if (ll == icol) continue;
-      const double dum = a[ll * K + icol];
-      const bool cand = !used(ll);
-      if (h0) {
-        const double base = (c0 == icol) ? 0.0 : a[ll * K + c0];
-        const double nv = fma(-pn0, dum, base);
-        a[ll * K + c0] = nv;
-        if (cand && c0 < mf && !used(c0)) consider(nv, ll, c0);
+    bidx = -1;
+    // eliminate this wave's rows; after the swap row irow holds old row icol
+    for (int l0 = w; l0 < mf; l0 += 4 * NW) {
+      double dm[4], b0[4], b1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ll = l0 + u * NW;
+        const int lr = ll < mf ? ll : w;
+        dm[u] = a[lr * K + icol];
+        b0[u] = h0 ? a[lr * K + c0] : 0.0;
+        b1[u] = h1 ? a[lr * K + c1] : 0.0;
       }
-      if (h1) {
-        const double base = (c1 == icol) ? 0.0 : a[ll * K + c1];
-        const double nv = fma(-pn1, dum, base);
-        a[ll * K + c1] = nv;
-        if (cand && c1 < mf && !used(c1)) consider(nv, ll, c1);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ll = l0 + u * NW;
+        if (ll >= mf || ll == icol) continue;
+        const bool sw = ll == irow;
+        const double dum = sw ? cpv : dm[u];
+        const bool cand = !used(ll);
+        if (h0) {
+          const double base = (c0 == icol) ? 0.0 : (sw ? cv0 : b0[u]);
+          const double nv = fma(-pn0, dum, base);
+          a[ll * K + c0] = nv;
+          if (cand && c0 < mf && !used(c0)) consider(nv, ll * 128 + c0);
+        }
+        if (h1) {
+          const double base = (c1 == icol) ? 0.0 : (sw ? cv1 : b1[u]);
+          const double nv = fma(-pn1, dum, base);
+          a[ll * K + c1] = nv;
+          if (cand && c1 < mf && !used(c1)) consider(nv, ll * 128 + c1);
+        }
       }
     }
-    wave_sync();
   }
   return 0;
 }
@@ -760,7 +848,7 @@ __device__ int gaussj_wave(double* a, int mf, int K) {
 struct PairState {
   double alamda, chisq, ochisq, iter_diff, dev_old, dev_new, chck;
   unsigned long long nvis;  // visible (point, facet) pairs over all evaluations
-  int niter, gj_err;
+  int niter;
 };
 
 }  // namespace
@@ -769,7 +857,7 @@ struct PairState {
 // Persistent LM fit kernel.
 // ---------------------------------------------------------------------------
 template <int NB, int NBS, bool YLDS>
-__global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void ps_fit_kernel(PsFitArgs A) {
   extern __shared__ double lds_dyn[];
   __shared__ PairState S;
   __shared__ int item_s;
@@ -840,12 +928,8 @@ __global__ __launch_bounds__(WG) void ps_fit_kernel(PsFitArgs A) {
             L.work[e] = vv;
           }
           __syncthreads();
-          if ((tid >> 6) == 0) {
-            const int e = gaussj_wave(L.work, mf, K);
-            if (tid == 0) S.gj_err = e;
-          }
+          const int err = gaussj_block<(NB > 4)>(L.work, mf, K, L.red);
           __syncthreads();
-          const int err = S.gj_err;
           if (err == 0 || err == 2)
             for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.work[q * K + mf];
           __syncthreads();
@@ -974,7 +1058,7 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
   }
 }
 
-// Gauss-Jordan of one damped system, by wave 0 (the parity tests' entry).
+// Gauss-Jordan of one damped system (the parity tests' entry).
 __global__ __launch_bounds__(WG) void ps_gaussj_kernel(int mf, const double* alpha, const double* beta, double lambda,
                                                        double* x, int* err) {
   extern __shared__ double lds_dyn[];
@@ -994,12 +1078,10 @@ __global__ __launch_bounds__(WG) void ps_gaussj_kernel(int mf, const double* alp
     work[e] = v;
   }
   __syncthreads();
-  if (tid < 64) {
-    const int e = gaussj_wave(work, mf, K);
-    wave_sync();
-    for (int j = tid; j < mf; j += 64) x[j] = work[j * K + mf];
-    if (tid == 0) *err = e;
-  }
+  const int e = K > 64 ? gaussj_block<true>(work, mf, K, work + K * K) : gaussj_block<false>(work, mf, K, work + K * K);
+  __syncthreads();
+  for (int j = tid; j < mf; j += WG) x[j] = work[j * K + mf];
+  if (tid == 0) *err = e;
 }
 
 __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2) {
@@ -1081,7 +1163,7 @@ hipError_t ps_phase_cycles(unsigned long long* out, bool reset) {
 hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda, double* x, int* err,
                             hipStream_t s) {
   const int K = mf + 1;
-  const size_t lds = (size_t)K * K * sizeof(double);
+  const size_t lds = ((size_t)K * K + 2 * NW) * sizeof(double);
   hipLaunchKernelGGL(ps_gaussj_kernel, dim3(1), dim3(WG), lds, s, mf, alpha, beta, lambda, x, err);
   return hipGetLastError();
 }
@@ -1089,6 +1171,20 @@ hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, dou
 hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hipStream_t s) {
   hipLaunchKernelGGL(ps_math_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, out, out2);
   return hipGetLastError();
+}
+
+int ps_fit_blocks_per_cu(int nb, int nbs, bool ylds, size_t lds) {
+#define PS_O(NB_, NBS_)                                                                                       \
+  if (nb == NB_ && nbs == NBS_) {                                                                             \
+    const void* fk = ylds ? reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, true>)                    \
+                          : reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, false>);                  \
+    int n = 0;                                                                                                \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fk, WG, lds) != hipSuccess) return 0;                \
+    return n;                                                                                                 \
+  }
+  PS_FOR_EACH_SHAPE(PS_O)
+#undef PS_O
+  return 0;
 }
 
 hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds) {

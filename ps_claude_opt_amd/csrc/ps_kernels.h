@@ -93,6 +93,7 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
 
 size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds);
 hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds);
+int ps_fit_blocks_per_cu(int nb, int nbs, bool ylds, size_t lds);  // resident workgroups per CU
 hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s);
 hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
                           double* jrows);
