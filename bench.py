@@ -146,8 +146,8 @@ def main() -> int:
         "data": "synthetic (seeded generator, ps_claude_opt_amd/synth.py)",
         "config": {
             "workload": f"plain FP64 scan {args.preset}: {info['ndata']} points in {info['n_curves']} relative "
-                        f"curves, lmax=6, {F} facets, {n_max} frequencies x 10 poles, {M} free parameters, "
-                        f"{info['n_iter_max']} LM iterations",
+                        f"curves, lmax={synth.PRESETS[args.preset].lmax}, {F} facets, {n_max} frequencies x 10 poles, "
+                        f"{M} free parameters, {info['n_iter_max']} LM iterations",
             "pairs_timed": pairs,
             "frequencies_per_step": batch,
             "variant": "plain (cold start)",

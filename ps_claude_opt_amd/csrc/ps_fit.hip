@@ -307,38 +307,53 @@ __device__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) 
   }
 }
 
+// Rotation angle of point pt and its phase-function terms (Start.cu:2095-2195).
+struct PointScalars {
+  double sf, cf, tt, scale, ff, d2, alph;
+};
+
+__device__ __forceinline__ PointScalars point_scalars(const PsDevProblem& P, const Inv& v, int pt) {
+  PointScalars q;
+  q.tt = P.tim[pt];
+  double f = fma(v.omega, q.tt, P.phi0);
+  f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
+  psm::sincos(f, &q.sf, &q.cf);
+  q.alph = P.alph[pt];
+  q.ff = psm::exp2(-1.44269504088896 * (q.alph * v.invD));
+  q.scale = fma(v.kph, q.alph, fma(v.A, q.ff, 1.0));
+  q.d2 = ((v.A * q.ff) * q.alph) * v.invD2;
+  return q;
+}
+
 // Phase 1 for one 16-point tile of one curve, executed by one wave.
 template <int NB, int NBS>
 __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int lp, int t, double* jbuf,
                                 double* ybuf, double* tsum, double* rowbuf) {
   const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
-  const Inv v = load_inv(L.inv);
   const int F = P.nfac, QS = P.qs, C = P.ncoef;
   const int pin = t * 16 + ps;
   const bool valid = pin < lp;
   const int pt = p0 + (valid ? pin : 0);
   (void)NB;
-  // point geometry: only the body-frame directions stay live in the loop
-  double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0}, scr[3];
-  double sf = 0.0, cf = 0.0, scale = 0.0, ff = 0.0, d2 = 0.0, alph = 0.0;
-  if (valid) {
-    const double tt = P.tim[pt];
-    double f = fma(v.omega, tt, P.phi0);
-    f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
-    psm::sincos(f, &sf, &cf);
-    alph = P.alph[pt];
-    ff = psm::exp2(-1.44269504088896 * (alph * v.invD));
-    scale = fma(v.kph, alph, fma(v.A, ff, 1.0));
-    d2 = ((v.A * ff) * alph) * v.invD2;
-    body_vectors(v, cf, sf, tt, P.ex[pt], P.ey[pt], P.ez[pt], g, scr, scr, scr);
-    body_vectors(v, cf, sf, tt, P.sx[pt], P.sy[pt], P.sz[pt], g0, scr, scr, scr);
+  // point geometry: only the body-frame directions stay live in the facet
+  // loop; the rest is recomputed (identically) for the row afterwards
+  double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0}, cl, cls;
+  {
+    const Inv v = load_inv(L.inv);
+    cl = v.cl;
+    cls = v.cls;
+    if (valid) {
+      double scr[3];
+      const PointScalars q = point_scalars(P, v, pt);
+      body_vectors(v, q.cf, q.sf, q.tt, P.ex[pt], P.ey[pt], P.ez[pt], g, scr, scr, scr);
+      body_vectors(v, q.cf, q.sf, q.tt, P.sx[pt], P.sy[pt], P.sz[pt], g0, scr, scr, scr);
+    }
   }
   double br = 0.0, t4 = 0.0, t5 = 0.0;
   double U0 = 0.0, U1 = 0.0, U2 = 0.0, V0 = 0.0, V1 = 0.0, V2 = 0.0;
   v4d gat[NBS];
 #pragma unroll
   for (int b = 0; b < NBS; ++b) gat[b] = v4d{0.0, 0.0, 0.0, 0.0};
-  const double cl = v.cl, cls = v.cls;
   // lane facet f = 4k + sl: normal/area at nrm[4f..4f+3], basis values of
   // shape columns 16b + ps at ytab[(16f + ps) * NBS + b].  Software
   // pipeline: the normals are loaded one step ahead, the basis values (LDS
@@ -412,10 +427,17 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int
   V2 = xsum16_32(V2);
   if (sl == 0) {
     double db[3], dl[3], dw[3], db0[3], dl0[3], dw0[3], gg[3];
-    const double tt = valid ? P.tim[pt] : 0.0;
-    body_vectors(v, cf, sf, tt, valid ? P.ex[pt] : 0.0, valid ? P.ey[pt] : 0.0, valid ? P.ez[pt] : 0.0, gg, db,
+    // reloaded through an index the optimiser cannot see through, so the
+    // loads are not merged with the ones before the facet loop
+    const int z = lane >> 6;
+    const Inv v = load_inv(L.inv + z);
+    const int pq = pt + z;
+    PointScalars q = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (valid) q = point_scalars(P, v, pq);
+    const double sf = q.sf, cf = q.cf, tt = q.tt, scale = q.scale, ff = q.ff, d2 = q.d2, alph = q.alph;
+    body_vectors(v, cf, sf, tt, valid ? P.ex[pq] : 0.0, valid ? P.ey[pq] : 0.0, valid ? P.ez[pq] : 0.0, gg, db,
                  dl, dw);
-    body_vectors(v, cf, sf, tt, valid ? P.sx[pt] : 0.0, valid ? P.sy[pt] : 0.0, valid ? P.sz[pt] : 0.0, gg,
+    body_vectors(v, cf, sf, tt, valid ? P.sx[pq] : 0.0, valid ? P.sy[pq] : 0.0, valid ? P.sz[pq] : 0.0, gg,
                  db0, dl0, dw0);
     auto tsum = [&](const double* dv, const double* dv0) {
       double r = dv[0] * U0;

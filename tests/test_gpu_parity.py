@@ -52,7 +52,7 @@ def test_device_math_bit_identical(scan):
     assert np.array_equal(c, osc[:, 1], equal_nan=True)
 
 
-@pytest.mark.parametrize("name", ["tiny", "small", "c1"])
+@pytest.mark.parametrize("name", ["tiny", "small", "c1", "c5"])
 def test_eval_bit_identical(scan, inputs, name):
     path, _ = inputs(name)
     O = _prep(scan, path)
@@ -138,6 +138,19 @@ def test_fit_pairs_bit_identical_sample(scan, inputs, preset, count):
     rng = np.random.default_rng(5)
     nl = list(rng.integers(1, O.info["n_max"] + 1, count))
     ml = list(rng.integers(1, 11, count))
+    cw = 0.8
+    R = scan.fit_pairs(nl, ml, cw, cw)
+    _compare_pairs(O, R, nl, ml, cw, cw)
+
+
+def test_fit_pairs_bit_identical_lmax8(scan, inputs):
+    """The widest kernel variant (lmax = 8: 81 coefficients, 86 free
+    parameters, 96-column rows, 512 facets) on a config-5-shaped asteroid
+    with fewer sparse points, so the oracle finishes in seconds."""
+    path, _ = inputs("c5_small", "c5", None, n_sparse=2000)
+    O = _prep(scan, path)
+    assert O.info["mfit"] > 64 and O.info["nfac"] == 512
+    nl, ml = [1, 700, 9000, 19999], [1, 5, 8, 10]
     cw = 0.8
     R = scan.fit_pairs(nl, ml, cw, cw)
     _compare_pairs(O, R, nl, ml, cw, cw)
