@@ -163,6 +163,52 @@ int ps_last_stats(const ps_ctx* ctx, double* kernel_ms, int* launches, long long
 /* CUDAUnprepare: frees the device copy of the problem. */
 int ps_unprepare(ps_ctx* ctx);
 
+/* ---- batches of asteroids (no counterpart in the reference, whose main()
+ * scans one asteroid per process; for callers that scan many asteroids
+ * with one persistent launch) ----
+ *
+ * A batch holds n problems that share the shape basis: the same nfac, lmax,
+ * mmax, free-parameter mask and facet tables (every period_search_in with
+ * the same nrows / lmax / mmax / fixed-parameter flags produces them), each
+ * with its own observations, scan range and controls.  Work items of all
+ * problems share one persistent launch, queued largest problem first; a
+ * pair's result does not depend on what else is in the launch.  The
+ * single-problem calls above address problem 0. */
+int ps_prepare_batch(ps_ctx* ctx, int n, const ps_problem* problems);
+int ps_prepare_files(ps_ctx* ctx, int n, const char* const* period_search_in_paths);
+int ps_n_problems(const ps_ctx* ctx);
+/* ps_info of problem i. */
+int ps_problem_info(const ps_ctx* ctx, int i, int* ints, double* dbls);
+/* Pair (n_list[k], m_list[k]) of problem prob[k], convexity rows weighted
+ * by conw_r[k], dev computed with conw_r_dev[k] (see ps_fit_pairs). */
+int ps_fit_items(ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
+                 const double* conw_r, const double* conw_r_dev, ps_pair_result* out);
+/* The convexity-weight loop of every problem (needs ps_prepare_files); the
+ * rounds of the problems still searching share one launch. */
+int ps_conw_search_batch(ps_ctx* ctx, double* conw_r /* [n] */);
+/* Scans frequencies n_start[i] .. n_start[i]+n_count[i]-1 of every problem
+ * i with conw_r[i]; the lines of problem i follow those of problems < i. */
+int ps_start_batch(ps_ctx* ctx, const int* n_start, const int* n_count, const double* conw_r, ps_line* out);
+
+/* ---- launch configuration (no counterpart in the reference, whose kernel
+ * launch shapes are compile-time constants; for callers tuning throughput
+ * on their own workloads) ----
+ *
+ * One LM fit runs in one workgroup of waves_per_pair wavefronts (4 or 8);
+ * wg_per_cu resident workgroups share a CU (0 = as many as fit).  Every
+ * configuration gives bit-identical results: it changes only which
+ * wavefront computes what, never the order of an operation chain.
+ * Defaults: 4 and 0 (the environment variables PS_WAVES_PER_PAIR and
+ * PS_WG_PER_CU override them at ps_open). */
+int ps_set_launch(ps_ctx* ctx, int waves_per_pair, int wg_per_cu);
+/* The configuration the next launch will use (after ps_prepare*). */
+int ps_get_launch(const ps_ctx* ctx, int* waves_per_pair, int* wg_per_cu);
+/* Times every configuration on the given work items (as ps_fit_items with
+ * conw_r_dev = conw_r), keeps the fastest for later calls on this context
+ * and reports it with its fit-kernel time in milliseconds. */
+int ps_autotune(ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
+                const double* conw_r, int* waves_per_pair, int* wg_per_cu, double* best_ms);
+
 /* ---- kernel-level entry points used by the parity tests ---- */
 
 /* One normal-equation evaluation (the reference's mrqcof, Start.cu:

@@ -54,8 +54,20 @@ def write_output(path: str, lines: Sequence[Line]) -> None:
                 f.write(format_line(L))
 
 
+_RESULT_FIELDS = ("reported", "niter", "dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "n_vis")
+
+
+def _results(out, n: int) -> np.ndarray:
+    res = np.zeros(n, dtype=[(f, "f8") for f in _RESULT_FIELDS])
+    for i in range(n):
+        for f in _RESULT_FIELDS:
+            res[i][f] = getattr(out[i], f)
+    return res
+
+
 class Scan:
-    """One device context holding one prepared problem."""
+    """One device context holding one prepared problem (or a batch of
+    problems sharing a shape basis)."""
 
     def __init__(self, device: int = 0):
         self._lib = _abi.load()
@@ -126,12 +138,84 @@ class Scan:
         dev = conw_r if conw_r_dev is None else conw_r_dev
         self._check(self._lib.ps_fit_pairs(self._h, len(n_arr), _abi.iptr(n_arr), _abi.iptr(m_arr),
                                            conw_r, dev, out))
-        names = ("reported", "niter", "dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "n_vis")
-        res = np.zeros(len(n_arr), dtype=[(n, "f8") for n in names])
-        for i in range(len(n_arr)):
-            for n in names:
-                res[i][n] = getattr(out[i], n)
-        return res
+        return _results(out, len(n_arr))
+
+    # -- batches of asteroids (one persistent launch for all) ---------------
+    def prepare_files(self, paths: Sequence[str]):
+        """Upload several period_search_in files sharing one shape basis."""
+        arr = (C.c_char_p * len(paths))(*[p.encode() for p in paths])
+        self._check(self._lib.ps_prepare_files(self._h, len(paths), arr))
+        self._load_info()
+        self.batch_info = [self._problem_info(i) for i in range(self._lib.ps_n_problems(self._h))]
+        return self
+
+    def _problem_info(self, i: int) -> dict:
+        ints = np.zeros(8, np.int32)
+        dbls = np.zeros(4, np.float64)
+        self._check(self._lib.ps_problem_info(self._h, i, _abi.iptr(ints), _abi.dptr(dbls)))
+        keys = ("ndata", "n_curves", "nfac", "ncoef", "ma", "mfit", "n_max", "n_iter_max")
+        d = {k: int(v) for k, v in zip(keys, ints)}
+        d.update(f_start=float(dbls[0]), f_step=float(dbls[1]), escl=float(dbls[2]), conw=float(dbls[3]))
+        return d
+
+    def conw_search_batch(self) -> np.ndarray:
+        out = np.zeros(len(self.batch_info))
+        self._check(self._lib.ps_conw_search_batch(self._h, _abi.dptr(out)))
+        return out
+
+    def start_batch(self, conw_r, n_start=None, n_count=None) -> List[List[Line]]:
+        """Scan every problem of the batch in one launch; lines per problem."""
+        n = len(self.batch_info)
+        ns = np.ascontiguousarray(np.ones(n) if n_start is None else n_start, np.int32)
+        nc = np.ascontiguousarray([b["n_max"] - int(s) + 1 for b, s in zip(self.batch_info, ns)]
+                                  if n_count is None else n_count, np.int32)
+        cw = np.ascontiguousarray(conw_r, np.float64)
+        total = int(nc.sum())
+        arr = (_abi.PsLine * max(total, 1))()
+        self._check(self._lib.ps_start_batch(self._h, _abi.iptr(ns), _abi.iptr(nc), _abi.dptr(cw), arr))
+        lines, k = [], 0
+        for i in range(n):
+            lines.append([Line(a.period_h, a.rms, a.chi2, a.dark_or_conw, a.lambda_deg, a.beta_deg)
+                          for a in arr[k:k + int(nc[i])]])
+            k += int(nc[i])
+        return lines
+
+    def fit_items(self, prob, n_list, m_list, conw_r, conw_r_dev=None):
+        """Pairs of several problems in one launch (per-item conwR)."""
+        p_arr = np.ascontiguousarray(prob, np.int32)
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        cw = np.ascontiguousarray(np.broadcast_to(conw_r, p_arr.shape), np.float64)
+        cd = np.ascontiguousarray(np.broadcast_to(conw_r if conw_r_dev is None else conw_r_dev, p_arr.shape),
+                                  np.float64)
+        out = (_abi.PsPairResult * max(len(p_arr), 1))()
+        self._check(self._lib.ps_fit_items(self._h, len(p_arr), _abi.iptr(p_arr), _abi.iptr(n_arr),
+                                           _abi.iptr(m_arr), _abi.dptr(cw), _abi.dptr(cd), out))
+        return _results(out, len(p_arr))
+
+    # -- launch configuration ------------------------------------------------
+    def set_launch(self, waves_per_pair: int = 4, wg_per_cu: int = 0):
+        self._check(self._lib.ps_set_launch(self._h, waves_per_pair, wg_per_cu))
+
+    def get_launch(self) -> dict:
+        w = C.c_int(0)
+        g = C.c_int(0)
+        self._check(self._lib.ps_get_launch(self._h, C.byref(w), C.byref(g)))
+        return dict(waves_per_pair=w.value, wg_per_cu=g.value)
+
+    def autotune(self, prob, n_list, m_list, conw_r) -> dict:
+        """Times every launch configuration on these items and keeps the
+        fastest (results do not depend on the configuration)."""
+        p_arr = np.ascontiguousarray(prob, np.int32)
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        cw = np.ascontiguousarray(np.broadcast_to(conw_r, p_arr.shape), np.float64)
+        w = C.c_int(0)
+        g = C.c_int(0)
+        ms = C.c_double(0.0)
+        self._check(self._lib.ps_autotune(self._h, len(p_arr), _abi.iptr(p_arr), _abi.iptr(n_arr), _abi.iptr(m_arr),
+                                          _abi.dptr(cw), C.byref(w), C.byref(g), C.byref(ms)))
+        return dict(waves_per_pair=w.value, wg_per_cu=g.value, kernel_ms=ms.value)
 
     def last_stats(self) -> dict:
         """Device time and work counters of the last fit call."""

@@ -77,6 +77,16 @@ SIGNATURES = {
     "ps_input_view": (C.c_int, [C.c_void_p, C.POINTER(PsProblem), _dp, _ip]),
     "ps_input_free": (None, [C.c_void_p]),
     "ps_select_best_pole": (C.c_int, [C.POINTER(PsPairResult)]),
+    "ps_prepare_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PsProblem)]),
+    "ps_prepare_files": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p)]),
+    "ps_n_problems": (C.c_int, [C.c_void_p]),
+    "ps_problem_info": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
+    "ps_fit_items": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, C.POINTER(PsPairResult)]),
+    "ps_conw_search_batch": (C.c_int, [C.c_void_p, _dp]),
+    "ps_start_batch": (C.c_int, [C.c_void_p, _ip, _ip, _dp, C.POINTER(PsLine)]),
+    "ps_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "ps_get_launch": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "ps_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp]),
 }
 
 _lib = None

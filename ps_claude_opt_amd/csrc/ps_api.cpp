@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -29,31 +30,41 @@ struct ps_input {
   ps::InputFile f;
 };
 
+// Host-side facts of one prepared problem.
+struct HostProb {
+  int ndata = 0, ncurves = 0, n_max = 0, n_iter_max = 0;
+  double f_start = 0, f_end = 0, f_step = 0, escl = 0, conw = 0;
+  bool have_file = false;
+};
+
 struct ps_ctx {
   int device = 0;
   int n_cu = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
-  // problem
+  // problems (one, or a batch sharing the shape basis)
   bool prepared = false;
-  bool have_file = false;
-  ps::InputFile file;
-  int ndata = 0, ncurves = 0, nfac = 0, ncoef = 0, ma = 0, mfit = 0, n_max = 0, n_iter_max = 0;
-  int max_lp = 0;
-  double f_start = 0, f_end = 0, f_step = 0, escl = 0, conw = 0;
-  PsDevProblem dp{};
+  std::vector<HostProb> hp;
+  std::vector<PsDevProblem> dps;  // host copies of the device descriptors
+  PsDevProblem* d_probs = nullptr;
+  int nfac = 0, ncoef = 0, ma = 0, mfit = 0;
   std::vector<void*> dbufs;
-  size_t lds = 0;
   long long stride = 0;
-  int grid_cap = 0;
-  int blocks_per_cu = 0;
-  // batch buffers
+  // launch configuration: wavefronts per workgroup (one pair per workgroup)
+  // and resident workgroups per CU (0 = as many as fit)
+  int waves = 4;
+  int wg_per_cu = 0;
+  size_t lds4 = 0, lds8 = 0;
+  int per_cu4 = 0, per_cu8 = 0;
+  // work-list buffers
   double* scratch = nullptr;
   size_t scratch_elems = 0;
   int* counter = nullptr;
+  int* d_items_p = nullptr;
   int* d_items_n = nullptr;
   int* d_items_m = nullptr;
+  double* d_items_cw = nullptr;
   double* d_out = nullptr;
   size_t items_cap = 0;
   // statistics of the last fit call
@@ -96,19 +107,25 @@ int upload(ps_ctx* c, const T* host, size_t n, const T** dev) {
 void free_problem(ps_ctx* c) {
   for (void* p : c->dbufs) hipFree(p);
   c->dbufs.clear();
+  c->d_probs = nullptr;
+  c->dps.clear();
+  c->hp.clear();
   c->prepared = false;
 }
 
-int ensure_batch(ps_ctx* c, int npairs, int grid) {
-  if ((size_t)npairs > c->items_cap) {
-    if (c->d_items_n) hipFree(c->d_items_n);
-    if (c->d_items_m) hipFree(c->d_items_m);
-    if (c->d_out) hipFree(c->d_out);
-    c->d_items_n = c->d_items_m = nullptr;
-    c->d_out = nullptr;
-    size_t cap = std::max<size_t>(npairs, 1024);
+int ensure_batch(ps_ctx* c, int n_items, int grid) {
+  if ((size_t)n_items > c->items_cap) {
+    for (void* q : {(void*)c->d_items_p, (void*)c->d_items_n, (void*)c->d_items_m, (void*)c->d_items_cw,
+                    (void*)c->d_out})
+      if (q) hipFree(q);
+    c->d_items_p = c->d_items_n = c->d_items_m = nullptr;
+    c->d_items_cw = c->d_out = nullptr;
+    c->items_cap = 0;
+    size_t cap = std::max<size_t>(n_items, 1024);
+    PS_HIP(c, hipMalloc(&c->d_items_p, cap * sizeof(int)));
     PS_HIP(c, hipMalloc(&c->d_items_n, cap * sizeof(int)));
     PS_HIP(c, hipMalloc(&c->d_items_m, cap * sizeof(int)));
+    PS_HIP(c, hipMalloc(&c->d_items_cw, cap * 2 * sizeof(double)));
     PS_HIP(c, hipMalloc(&c->d_out, cap * PS_OUT_STRIDE * sizeof(double)));
     c->items_cap = cap;
   }
@@ -116,6 +133,7 @@ int ensure_batch(ps_ctx* c, int npairs, int grid) {
   if (need > c->scratch_elems) {
     if (c->scratch) hipFree(c->scratch);
     c->scratch = nullptr;
+    c->scratch_elems = 0;
     PS_HIP(c, hipMalloc(&c->scratch, need * sizeof(double)));
     c->scratch_elems = need;
   }
@@ -123,18 +141,20 @@ int ensure_batch(ps_ctx* c, int npairs, int grid) {
   return PS_OK;
 }
 
-PsFitArgs make_args(ps_ctx* c, double conw_r, double conw_r_dev, int n_items) {
-  PsFitArgs A;
-  A.P = c->dp;
-  A.conw_r2_rows = conw_r * conw_r;
-  A.conw_r_dev = conw_r_dev;
+PsFitArgs make_args(ps_ctx* c, int n_items) {
+  PsFitArgs A{};
+  A.P0 = c->dps[0];
+  A.probs = c->d_probs;
+  A.item_p = c->d_items_p;
   A.item_n = c->d_items_n;
   A.item_m = c->d_items_m;
+  A.item_cw = c->d_items_cw;
   A.n_items = n_items;
   A.counter = c->counter;
   A.scratch = c->scratch;
   A.scratch_stride = c->stride;
   A.out = c->d_out;
+  A.batch = c->dps.size() > 1 ? 1 : 0;
   return A;
 }
 
@@ -144,6 +164,324 @@ int best_pole(const ps_pair_result* r) {
   for (int p = 0; p < PS_N_POLES; ++p)
     if (r[p].reported == 1 && (best < 0 || (!std::isnan(r[p].dev) && !(r[p].dev >= r[best].dev)))) best = p;
   return best;
+}
+
+// Validation of one problem; writes mfit.
+int check_problem(ps_ctx* c, const ps_problem* p, int* mfit_out) {
+  const int N = p->ndata, L = p->n_curves, F = p->nfac, C = p->ncoef;
+  if (N <= 0 || L <= 0 || F <= 0 || F % 4 || C <= 1 || p->lmax < 0 || p->mmax < 0 || p->mmax > p->lmax)
+    return fail(c, PS_E_ARG, "bad problem sizes");
+  int count = 0;
+  for (int i = 0; i < L; ++i) {
+    if (p->l_points[i] <= 0) return fail(c, PS_E_ARG, "empty lightcurve");
+    if (!p->in_rel[i]) return fail(c, PS_E_UNSUPPORTED, "calibrated (absolute) lightcurves are not supported");
+    count += p->l_points[i];
+  }
+  if (count != N) return fail(c, PS_E_ARG, "l_points do not sum to ndata");
+  const int ma = C + 8;
+  // free parameters must be one contiguous run starting at parameter 2
+  int mfit = 0, lastma = 0;
+  for (int i = 0; i < ma; ++i)
+    if (p->ia[i]) {
+      ++mfit;
+      lastma = i + 1;
+    }
+  if (p->ia[0]) return fail(c, PS_E_UNSUPPORTED, "the size-scale parameter must be fixed (relative data)");
+  for (int i = 1; i < lastma; ++i)
+    if (!p->ia[i]) return fail(c, PS_E_UNSUPPORTED, "free parameters must be contiguous from parameter 2");
+  if (mfit < 1) return fail(c, PS_E_ARG, "no free parameters");
+  const int K = mfit + 1;
+  const int nb = std::max((C + 7 + 15) / 16, (K + 15) / 16);
+  if (nb > 6 || (C - 1 + 15) / 16 < nb - 1)
+    return fail(c, PS_E_UNSUPPORTED, "more than 96 Jacobian columns (lmax too large)");
+  *mfit_out = mfit;
+  return PS_OK;
+}
+
+bool same_doubles(const double* a, const double* b, size_t n) { return std::memcmp(a, b, n * sizeof(double)) == 0; }
+
+// Problems of one launch must share the shape basis: sizes, free-parameter
+// map and the facet tables, bit for bit.
+int check_shared(ps_ctx* c, const ps_problem* a, const ps_problem* b) {
+  const int F = a->nfac, C = a->ncoef;
+  if (b->nfac != F || b->ncoef != C || b->lmax != a->lmax || b->mmax != a->mmax)
+    return fail(c, PS_E_UNSUPPORTED, "problems of a batch must share nfac, lmax and mmax");
+  if (std::memcmp(a->ia, b->ia, (size_t)(C + 8) * sizeof(int)) != 0)
+    return fail(c, PS_E_UNSUPPORTED, "problems of a batch must share the free-parameter mask");
+  const size_t npl = (size_t)(a->mmax + 1) * (a->lmax + 1) * F, nfc = (size_t)(a->mmax + 1) * F;
+  if (!same_doubles(a->normal, b->normal, 3 * (size_t)F) || !same_doubles(a->d_area, b->d_area, F) ||
+      !same_doubles(a->pleg, b->pleg, npl) || !same_doubles(a->fc, b->fc, nfc) || !same_doubles(a->fs, b->fs, nfc) ||
+      !same_doubles(a->d_sphere, b->d_sphere, (size_t)C * F))
+    return fail(c, PS_E_UNSUPPORTED, "problems of a batch must share the facet tables");
+  return PS_OK;
+}
+
+// Uploads n problems (the shared tables once, from problem 0) and sizes the
+// launch.  conw[i] / have_file[i] describe period_search_in headers.
+int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* conw, const bool* have_file) {
+  PS_HIP(c, hipSetDevice(c->device));
+  free_problem(c);
+  if (n < 1) return fail(c, PS_E_ARG, "no problems");
+  int mfit = 0;
+  for (int i = 0; i < n; ++i) {
+    int mf = 0;
+    int rc = check_problem(c, ps_[i], &mf);
+    if (rc) return rc;
+    if (i == 0) mfit = mf;
+    if (i > 0 && (rc = check_shared(c, ps_[0], ps_[i])) != PS_OK) return rc;
+  }
+  const ps_problem* p0 = ps_[0];
+  const int F = p0->nfac, C = p0->ncoef, ma = C + 8, K = mfit + 1;
+  const int nb = std::max((C + 7 + 15) / 16, (K + 15) / 16);
+  const int nbs = (C - 1 + 15) / 16;
+  PsDevProblem shared{};
+  int rc;
+#define UP(dst, ptr, cnt) \
+  if ((rc = upload(c, ptr, cnt, &dst)) != PS_OK) return rc
+  {
+    std::vector<double> yl((size_t)F * 16 * nbs, 0.0);
+    for (int f = 0; f < F; ++f)
+      for (int q = 0; q <= C - 2; ++q)
+        yl[((size_t)f * 16 + (q & 15)) * nbs + (q >> 4)] = p0->d_sphere[(size_t)(q + 1) * F + f];
+    std::vector<int> cmap;
+    for (int i = 0; i < ma; ++i)
+      if (p0->ia[i]) cmap.push_back(i);
+    UP(shared.nx, p0->normal, F);
+    UP(shared.ny, p0->normal + F, F);
+    UP(shared.nz, p0->normal + 2 * F, F);
+    UP(shared.darea, p0->d_area, F);
+    UP(shared.pleg, p0->pleg, (size_t)(p0->mmax + 1) * (p0->lmax + 1) * F);
+    UP(shared.fc, p0->fc, (size_t)(p0->mmax + 1) * F);
+    UP(shared.fs, p0->fs, (size_t)(p0->mmax + 1) * F);
+    UP(shared.yl, yl.data(), yl.size());
+    UP(shared.cmap, cmap.data(), cmap.size());
+  }
+  shared.nfac = F;
+  shared.lmax = p0->lmax;
+  shared.mmax = p0->mmax;
+  shared.ncoef = C;
+  shared.ma = ma;
+  shared.mfit = mfit;
+  shared.nb = nb;
+  shared.qs = 16 * nb;
+  shared.nbs = nbs;
+  c->stride = 0;
+  for (int pi = 0; pi < n; ++pi) {
+    const ps_problem* p = ps_[pi];
+    const int N = p->ndata, L = p->n_curves;
+    PsDevProblem d = shared;
+    HostProb h;
+    std::vector<double> alph(N), s2w(N);
+    for (int i = 0; i < N; ++i) {
+      const double dd = std::fma(p->ee[2 * N + i], p->ee0[2 * N + i],
+                                 std::fma(p->ee[N + i], p->ee0[N + i], p->ee[i] * p->ee0[i]));
+      alph[i] = std::acos(std::fmin(1.0, std::fmax(-1.0, dd)));
+      s2w[i] = p->sigr2[i] * p->weight[i];
+    }
+    std::vector<int> off(L + 1, 0), toff(L + 1, 0), tcv;
+    for (int i = 0; i < L; ++i) {
+      off[i + 1] = off[i] + p->l_points[i];
+      const int nt = (p->l_points[i] + 15) / 16;
+      toff[i + 1] = toff[i] + nt;
+      for (int t = 0; t < nt; ++t) tcv.push_back(i);
+    }
+    UP(d.tim, p->tim, N);
+    UP(d.ex, p->ee, N);
+    UP(d.ey, p->ee + N, N);
+    UP(d.ez, p->ee + 2 * N, N);
+    UP(d.sx, p->ee0, N);
+    UP(d.sy, p->ee0 + N, N);
+    UP(d.sz, p->ee0 + 2 * N, N);
+    UP(d.alph, alph.data(), N);
+    UP(d.bright, p->brightness, N);
+    UP(d.sig, p->sig, N);
+    UP(d.s2w, s2w.data(), N);
+    UP(d.curve_off, off.data(), L + 1);
+    UP(d.tile_off, toff.data(), L + 1);
+    UP(d.tile_cv, tcv.data(), tcv.size());
+    UP(d.cg_first, p->cg_first, C);
+    d.ncurves = L;
+    d.ndata = N;
+    d.ntiles = (int)tcv.size();
+    for (int i = 0; i < 3; ++i) d.par[i] = p->par[i];
+    d.lcl = std::log(p->cl);
+    d.phi0 = p->phi0;
+    d.alamda_start = p->alamda_start;
+    d.alamda_incr = p->alamda_incr;
+    d.alamda_incrr = p->alamda_incrr;
+    // start_CUDA.cu:855-869
+    int n_iter_min = p->n_iter_min;
+    d.n_iter_max = 0;
+    d.iter_diff_max = -1;
+    if (p->stop_condition > 1) {
+      d.n_iter_max = (int)p->stop_condition;
+      d.iter_diff_max = 0;
+      n_iter_min = 0;
+    }
+    if (p->stop_condition < 1) {
+      d.n_iter_max = kMaxIter;
+      d.iter_diff_max = p->stop_condition;
+    }
+    d.n_iter_min = n_iter_min;
+    d.f_start = p->f_start;
+    d.f_step = p->f_step;
+    for (int i = 0; i < PS_N_POLES; ++i) {
+      d.beta_pole[i] = p->beta_pole[i];
+      d.lambda_pole[i] = p->lambda_pole[i];
+    }
+    h.ndata = N;
+    h.ncurves = L;
+    h.f_start = p->f_start;
+    h.f_end = p->f_end;
+    h.f_step = p->f_step;
+    h.n_max = (int)((p->f_start - p->f_end) / p->f_step) + 1;
+    h.n_iter_max = d.n_iter_max;
+    h.escl = p->escl;
+    h.conw = conw ? conw[pi] : 0.0;
+    h.have_file = have_file ? have_file[pi] : false;
+    c->dps.push_back(d);
+    c->hp.push_back(h);
+    c->stride = std::max(c->stride, (long long)ps_slab(d).stride);
+  }
+#undef UP
+  {
+    void* q = nullptr;
+    PS_HIP(c, hipMalloc(&q, sizeof(PsDevProblem) * n));
+    c->dbufs.push_back(q);
+    PS_HIP(c, hipMemcpy(q, c->dps.data(), sizeof(PsDevProblem) * n, hipMemcpyHostToDevice));
+    c->d_probs = static_cast<PsDevProblem*>(q);
+  }
+  c->nfac = F;
+  c->ncoef = C;
+  c->ma = ma;
+  c->mfit = mfit;
+  const size_t lds_cap = 160 * 1024;
+  c->lds4 = psw4::ps_fit_lds_bytes(c->dps[0]);
+  c->lds8 = psw8::ps_fit_lds_bytes(c->dps[0]);
+  if (c->lds4 > lds_cap) return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
+  PS_HIP(c, psw4::ps_set_fit_attributes(nb, nbs, c->lds4));
+  c->per_cu4 = psw4::ps_fit_blocks_per_cu(nb, nbs, c->lds4);
+  if (c->per_cu4 < 1) return fail(c, PS_E_HIP, "fit kernel cannot be resident (occupancy query failed)");
+  c->per_cu8 = 0;
+  if (c->lds8 <= lds_cap && psw8::ps_set_fit_attributes(nb, nbs, c->lds8) == hipSuccess)
+    c->per_cu8 = psw8::ps_fit_blocks_per_cu(nb, nbs, c->lds8);
+  c->prepared = true;
+  return PS_OK;
+}
+
+// Fits a list of work items in one persistent launch.  Items are queued
+// largest problem first (the work of a pair grows with the problem's point
+// count), results come back in the caller's order.
+int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list, const double* cw_rows,
+              const double* cw_dev, ps_pair_result* out) {
+  c->last_ms = 0;
+  c->last_launches = 0;
+  c->last_pairs = c->last_vis = c->last_iters = c->last_evals = 0;
+  if (n_items == 0) return PS_OK;
+  const int np = (int)c->dps.size();
+  for (int i = 0; i < n_items; ++i) {
+    const int pi = prob ? prob[i] : 0;
+    if (pi < 0 || pi >= np) return fail(c, PS_E_ARG, "problem index out of range");
+    if (n_list[i] < 1 || m_list[i] < 1 || m_list[i] > PS_N_POLES) return fail(c, PS_E_ARG, "pair index out of range");
+  }
+  PS_HIP(c, hipSetDevice(c->device));
+  std::vector<int> order(n_items);
+  for (int i = 0; i < n_items; ++i) order[i] = i;
+  if (prob && np > 1)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return c->hp[prob[a]].ndata > c->hp[prob[b]].ndata; });
+  std::vector<int> ip(n_items), in(n_items), im(n_items);
+  std::vector<double> icw(2 * (size_t)n_items);
+  for (int k = 0; k < n_items; ++k) {
+    const int i = order[k];
+    ip[k] = prob ? prob[i] : 0;
+    in[k] = n_list[i];
+    im[k] = m_list[i];
+    icw[2 * k] = cw_rows[i];
+    icw[2 * k + 1] = cw_dev[i];
+  }
+  const bool w8 = c->waves == 8 && c->per_cu8 > 0;
+  const int per_cu = w8 ? c->per_cu8 : c->per_cu4;
+  const int wpc = c->wg_per_cu > 0 ? std::min(c->wg_per_cu, per_cu) : per_cu;
+  const int grid = std::min(n_items, c->n_cu * wpc);
+  int rc = ensure_batch(c, n_items, grid);
+  if (rc) return rc;
+  PS_HIP(c, hipMemcpyAsync(c->d_items_p, ip.data(), n_items * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  PS_HIP(c, hipMemcpyAsync(c->d_items_n, in.data(), n_items * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  PS_HIP(c, hipMemcpyAsync(c->d_items_m, im.data(), n_items * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  PS_HIP(c, hipMemcpyAsync(c->d_items_cw, icw.data(), icw.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int zero = 0;
+  PS_HIP(c, hipMemcpyAsync(c->counter, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  PsFitArgs A = make_args(c, n_items);
+  PS_HIP(c, hipEventRecord(c->ev0, c->stream));
+  PS_HIP(c, w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream));
+  PS_HIP(c, hipEventRecord(c->ev1, c->stream));
+  std::vector<double> res((size_t)n_items * PS_OUT_STRIDE);
+  PS_HIP(c, hipMemcpyAsync(res.data(), c->d_out, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PS_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  PS_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  c->last_launches = 1;
+  if (std::getenv("PS_PHASE_TIMING")) {
+    unsigned long long cyc[16];
+    if ((w8 ? psw8::ps_phase_cycles(cyc, true) : psw4::ps_phase_cycles(cyc, true)) == hipSuccess) {
+      std::fprintf(stderr, "phase cycles (thread 0 of each workgroup, summed):");
+      for (int i = 0; i < 9; ++i) std::fprintf(stderr, " %d:%.4g", i, (double)cyc[i]);
+      std::fprintf(stderr, "\n");
+    }
+  }
+  for (int k = 0; k < n_items; ++k) {
+    const double* o = &res[(size_t)k * PS_OUT_STRIDE];
+    ps_pair_result& r = out[order[k]];
+    r.dev = o[0];
+    r.dark = o[1];
+    r.period = o[2];
+    r.lambda_deg = o[3];
+    r.beta_deg = o[4];
+    r.chisq = o[5];
+    r.niter = (int)o[6];
+    r.reported = (int)o[7];
+    r.n_vis = (long long)o[8];
+    c->last_vis += r.n_vis;
+    c->last_iters += r.niter;
+    c->last_evals += r.niter > 0 ? r.niter + 1 : 0;
+  }
+  c->last_pairs = n_items;
+  return PS_OK;
+}
+
+// Output lines of a scan from its pair results (start_CUDA.cu:1200-1229).
+void make_lines(const HostProb& h, int n_start, int n_count, double conw_r, const ps_pair_result* r, ps_line* out) {
+  for (int i = 0; i < n_count; ++i) {
+    ps_line& L = out[i];
+    const int b = best_pole(&r[(size_t)i * PS_N_POLES]);
+    if (b < 0) {
+      L = ps_line{NAN, NAN, NAN, NAN, NAN, NAN};
+      continue;
+    }
+    const ps_pair_result& x = r[(size_t)i * PS_N_POLES + b];
+    L.period_h = 24 * x.period;
+    L.rms = x.dev;
+    L.chi2 = x.dev * x.dev * h.ndata;
+    L.dark_or_conw = (n_start + i == 1) ? conw_r * h.escl * h.escl : x.dark;
+    L.lambda_deg = x.lambda_deg;
+    L.beta_deg = x.beta_deg;
+  }
+}
+
+// Conw decision of one round from its 10 x 10 results (start_CUDA.cu:
+// 700-789): mean dark-facet percentage of the best poles.
+void conw_decide(const ps_pair_result* r, double* conw_r, int* new_conw) {
+  const int tests = 10;
+  double sum_dark = 0.0;
+  for (int n = 0; n < tests; ++n) {
+    const int b = best_pole(&r[n * PS_N_POLES]);
+    if (b >= 0) sum_dark = sum_dark + r[n * PS_N_POLES + b].dark;
+  }
+  const double ave = sum_dark / tests;
+  if (ave < 1.0) *new_conw = 1;
+  if (ave >= 1.0) *conw_r = *conw_r * 2;
 }
 
 }  // namespace
@@ -199,6 +537,8 @@ int ps_open(int device, ps_ctx** out) {
     return PS_E_HIP;
   }
   c->n_cu = prop.multiProcessorCount;
+  if (const char* w = std::getenv("PS_WAVES_PER_PAIR")) c->waves = std::atoi(w) == 8 ? 8 : 4;
+  if (const char* g = std::getenv("PS_WG_PER_CU")) c->wg_per_cu = std::max(0, std::atoi(g));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -216,9 +556,8 @@ int ps_close(ps_ctx* c) {
   free_problem(c);
   if (c->scratch) hipFree(c->scratch);
   if (c->counter) hipFree(c->counter);
-  if (c->d_items_n) hipFree(c->d_items_n);
-  if (c->d_items_m) hipFree(c->d_items_m);
-  if (c->d_out) hipFree(c->d_out);
+  for (void* q : {(void*)c->d_items_p, (void*)c->d_items_n, (void*)c->d_items_m, (void*)c->d_items_cw, (void*)c->d_out})
+    if (q) hipFree(q);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -228,237 +567,77 @@ int ps_close(ps_ctx* c) {
 
 int ps_prepare(ps_ctx* c, const ps_problem* p) {
   if (!c || !p) return PS_E_ARG;
-  PS_HIP(c, hipSetDevice(c->device));
-  free_problem(c);
-  const int N = p->ndata, L = p->n_curves, F = p->nfac, C = p->ncoef;
-  if (N <= 0 || L <= 0 || F <= 0 || F % 4 || C <= 1 || p->lmax < 0 || p->mmax < 0 || p->mmax > p->lmax)
-    return fail(c, PS_E_ARG, "bad problem sizes");
-  int count = 0;
-  for (int i = 0; i < L; ++i) {
-    if (p->l_points[i] <= 0) return fail(c, PS_E_ARG, "empty lightcurve");
-    if (!p->in_rel[i]) return fail(c, PS_E_UNSUPPORTED, "calibrated (absolute) lightcurves are not supported");
-    count += p->l_points[i];
+  return prepare_many(c, 1, &p, nullptr, nullptr);
+}
+
+int ps_prepare_batch(ps_ctx* c, int n, const ps_problem* problems) {
+  if (!c || n < 1 || !problems) return PS_E_ARG;
+  std::vector<const ps_problem*> v(n);
+  for (int i = 0; i < n; ++i) v[i] = &problems[i];
+  return prepare_many(c, n, v.data(), nullptr, nullptr);
+}
+
+int ps_prepare_files(ps_ctx* c, int n, const char* const* paths) {
+  if (!c || n < 1 || !paths) return PS_E_ARG;
+  std::vector<ps::InputFile> files(n);
+  std::vector<ps_problem> views(n);
+  std::vector<const ps_problem*> v(n);
+  std::vector<double> conw(n);
+  std::unique_ptr<bool[]> hf(new bool[n]);
+  for (int i = 0; i < n; ++i) {
+    if (!paths[i]) return PS_E_ARG;
+    std::string e;
+    const int rc = ps::read_input(paths[i], files[i], e);
+    if (rc) return fail(c, rc, std::string(paths[i]) + ": " + e);
+    views[i] = files[i].view();
+    views[i].stop_condition = files[i].stop_condition;
+    v[i] = &views[i];
+    conw[i] = files[i].conw;
+    hf[i] = true;
   }
-  if (count != N) return fail(c, PS_E_ARG, "l_points do not sum to ndata");
-  const int ma = C + 8;
-  // free parameters must be one contiguous run starting at parameter 2
-  int mfit = 0, lastma = 0;
-  for (int i = 0; i < ma; ++i)
-    if (p->ia[i]) {
-      ++mfit;
-      lastma = i + 1;
-    }
-  if (p->ia[0]) return fail(c, PS_E_UNSUPPORTED, "the size-scale parameter must be fixed (relative data)");
-  for (int i = 1; i < lastma; ++i)
-    if (!p->ia[i]) return fail(c, PS_E_UNSUPPORTED, "free parameters must be contiguous from parameter 2");
-  if (mfit < 1) return fail(c, PS_E_ARG, "no free parameters");
-  const int K = mfit + 1;
-  const int nb = std::max((C + 7 + 15) / 16, (K + 15) / 16);
-  if (nb > 6 || (C - 1 + 15) / 16 < nb - 1) return fail(c, PS_E_UNSUPPORTED, "more than 96 Jacobian columns (lmax too large)");
-  // host-derived tables
-  const int nbs = (C - 1 + 15) / 16;
-  std::vector<double> alph(N), s2w(N), yl((size_t)F * 16 * nbs, 0.0);
-  for (int i = 0; i < N; ++i) {
-    const double d = std::fma(p->ee[2 * N + i], p->ee0[2 * N + i],
-                              std::fma(p->ee[N + i], p->ee0[N + i], p->ee[i] * p->ee0[i]));
-    alph[i] = std::acos(std::fmin(1.0, std::fmax(-1.0, d)));
-    s2w[i] = p->sigr2[i] * p->weight[i];
-  }
-  for (int f = 0; f < F; ++f)
-    for (int q = 0; q <= C - 2; ++q) yl[((size_t)f * 16 + (q & 15)) * nbs + (q >> 4)] = p->d_sphere[(size_t)(q + 1) * F + f];
-  std::vector<int> off(L + 1, 0), toff(L + 1, 0), tcv, cmap;
-  int maxlp = 3;
-  for (int i = 0; i < L; ++i) {
-    off[i + 1] = off[i] + p->l_points[i];
-    maxlp = std::max(maxlp, p->l_points[i]);
-    const int nt = (p->l_points[i] + 15) / 16;
-    toff[i + 1] = toff[i] + nt;
-    for (int t = 0; t < nt; ++t) tcv.push_back(i);
-  }
-  for (int i = 0; i < ma; ++i)
-    if (p->ia[i]) cmap.push_back(i);
-  PsDevProblem& d = c->dp;
-  d = PsDevProblem{};
-  int rc;
-#define UP(field, ptr, n) \
-  if ((rc = upload(c, ptr, n, &d.field)) != PS_OK) return rc
-  UP(tim, p->tim, N);
-  UP(ex, p->ee, N);
-  UP(ey, p->ee + N, N);
-  UP(ez, p->ee + 2 * N, N);
-  UP(sx, p->ee0, N);
-  UP(sy, p->ee0 + N, N);
-  UP(sz, p->ee0 + 2 * N, N);
-  UP(alph, alph.data(), N);
-  UP(bright, p->brightness, N);
-  UP(sig, p->sig, N);
-  UP(s2w, s2w.data(), N);
-  UP(curve_off, off.data(), L + 1);
-  UP(tile_off, toff.data(), L + 1);
-  UP(tile_cv, tcv.data(), tcv.size());
-  UP(nx, p->normal, F);
-  UP(ny, p->normal + F, F);
-  UP(nz, p->normal + 2 * F, F);
-  UP(darea, p->d_area, F);
-  UP(pleg, p->pleg, (size_t)(p->mmax + 1) * (p->lmax + 1) * F);
-  UP(fc, p->fc, (size_t)(p->mmax + 1) * F);
-  UP(fs, p->fs, (size_t)(p->mmax + 1) * F);
-  UP(yl, yl.data(), yl.size());
-  UP(cg_first, p->cg_first, C);
-  UP(cmap, cmap.data(), cmap.size());
-#undef UP
-  d.ncurves = L;
-  d.ndata = N;
-  d.ntiles = (int)tcv.size();
-  d.nfac = F;
-  d.lmax = p->lmax;
-  d.mmax = p->mmax;
-  d.ncoef = C;
-  d.ma = ma;
-  d.mfit = mfit;
-  d.nb = nb;
-  d.qs = 16 * nb;
-  d.nbs = nbs;
-  for (int i = 0; i < 3; ++i) d.par[i] = p->par[i];
-  d.lcl = std::log(p->cl);
-  d.phi0 = p->phi0;
-  d.alamda_start = p->alamda_start;
-  d.alamda_incr = p->alamda_incr;
-  d.alamda_incrr = p->alamda_incrr;
-  // start_CUDA.cu:855-869
-  int n_iter_min = p->n_iter_min;
-  d.n_iter_max = 0;
-  d.iter_diff_max = -1;
-  if (p->stop_condition > 1) {
-    d.n_iter_max = (int)p->stop_condition;
-    d.iter_diff_max = 0;
-    n_iter_min = 0;
-  }
-  if (p->stop_condition < 1) {
-    d.n_iter_max = kMaxIter;
-    d.iter_diff_max = p->stop_condition;
-  }
-  d.n_iter_min = n_iter_min;
-  d.f_start = p->f_start;
-  d.f_step = p->f_step;
-  for (int i = 0; i < PS_N_POLES; ++i) {
-    d.beta_pole[i] = p->beta_pole[i];
-    d.lambda_pole[i] = p->lambda_pole[i];
-  }
-  c->ndata = N;
-  c->ncurves = L;
-  c->nfac = F;
-  c->ncoef = C;
-  c->ma = ma;
-  c->mfit = mfit;
-  c->f_start = p->f_start;
-  c->f_end = p->f_end;
-  c->f_step = p->f_step;
-  c->n_max = (int)((p->f_start - p->f_end) / p->f_step) + 1;
-  c->n_iter_max = d.n_iter_max;
-  c->escl = p->escl;
-  c->max_lp = maxlp;
-  c->stride = (long long)ps_slab(d).stride;
-  // The shape basis is read from L2 by default, which leaves room for two
-  // workgroups per CU; PS_BASIS_LDS=1 stages it in LDS instead (one
-  // workgroup per CU) when it fits.
-  const size_t lds_cap = 160 * 1024;
-  const char* bl = std::getenv("PS_BASIS_LDS");
-  d.ylds = (bl && bl[0] == '1' && ps_fit_lds_bytes(d, true) <= lds_cap) ? 1 : 0;
-  c->lds = ps_fit_lds_bytes(d, d.ylds != 0);
-  if (c->lds > lds_cap) return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
-  PS_HIP(c, ps_set_fit_attributes(nb, d.nbs, d.ylds != 0, c->lds));
-  const int per_cu = ps_fit_blocks_per_cu(nb, d.nbs, d.ylds != 0, c->lds);
-  if (per_cu < 1) return fail(c, PS_E_HIP, "fit kernel cannot be resident (occupancy query failed)");
-  c->grid_cap = c->n_cu * per_cu;
-  c->blocks_per_cu = per_cu;
-  c->prepared = true;
-  return PS_OK;
+  return prepare_many(c, n, v.data(), conw.data(), hf.get());
 }
 
 int ps_prepare_file(ps_ctx* c, const char* path) {
   if (!c || !path) return PS_E_ARG;
-  c->file = ps::InputFile();
-  std::string e;
-  int rc = ps::read_input(path, c->file, e);
-  if (rc) return fail(c, rc, e);
-  c->have_file = true;
-  ps_problem p = c->file.view();
-  p.stop_condition = c->file.stop_condition;
-  rc = ps_prepare(c, &p);
-  if (rc) return rc;
-  c->conw = c->file.conw;
-  return PS_OK;
+  return ps_prepare_files(c, 1, &path);
 }
 
-int ps_info(const ps_ctx* c, int* ints, double* dbls) {
+int ps_n_problems(const ps_ctx* c) { return (c && c->prepared) ? (int)c->hp.size() : 0; }
+
+int ps_problem_info(const ps_ctx* c, int i, int* ints, double* dbls) {
   if (!c || !c->prepared) return PS_E_STATE;
+  if (i < 0 || i >= (int)c->hp.size()) return PS_E_ARG;
+  const HostProb& h = c->hp[i];
   if (ints) {
-    const int v[8] = {c->ndata, c->ncurves, c->nfac, c->ncoef, c->ma, c->mfit, c->n_max, c->n_iter_max};
+    const int v[8] = {h.ndata, h.ncurves, c->nfac, c->ncoef, c->ma, c->mfit, h.n_max, h.n_iter_max};
     std::memcpy(ints, v, sizeof v);
   }
   if (dbls) {
-    const double v[4] = {c->f_start, c->f_step, c->escl, c->have_file ? c->conw : 0.0};
+    const double v[4] = {h.f_start, h.f_step, h.escl, h.have_file ? h.conw : 0.0};
     std::memcpy(dbls, v, sizeof v);
   }
   return PS_OK;
 }
 
+int ps_info(const ps_ctx* c, int* ints, double* dbls) { return ps_problem_info(c, 0, ints, dbls); }
+
 int ps_fit_pairs(ps_ctx* c, int npairs, const int* n_list, const int* m_list, double conw_r, double conw_r_dev,
                  ps_pair_result* out) {
   if (!c || npairs < 0 || (npairs && (!n_list || !m_list || !out))) return PS_E_ARG;
   if (!c->prepared) return fail(c, PS_E_STATE, "ps_fit_pairs before ps_prepare");
-  for (int i = 0; i < npairs; ++i)
-    if (n_list[i] < 1 || m_list[i] < 1 || m_list[i] > PS_N_POLES) return fail(c, PS_E_ARG, "pair index out of range");
-  c->last_ms = 0;
-  c->last_launches = 0;
-  c->last_pairs = c->last_vis = c->last_iters = c->last_evals = 0;
-  if (npairs == 0) return PS_OK;
-  PS_HIP(c, hipSetDevice(c->device));
-  const int grid = std::min(npairs, c->grid_cap);
-  int rc = ensure_batch(c, npairs, grid);
-  if (rc) return rc;
-  PS_HIP(c, hipMemcpyAsync(c->d_items_n, n_list, npairs * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  PS_HIP(c, hipMemcpyAsync(c->d_items_m, m_list, npairs * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  const int zero = 0;
-  PS_HIP(c, hipMemcpyAsync(c->counter, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-  PsFitArgs A = make_args(c, conw_r, conw_r_dev, npairs);
-  PS_HIP(c, hipEventRecord(c->ev0, c->stream));
-  PS_HIP(c, ps_launch_fit(A, grid, c->stream));
-  PS_HIP(c, hipEventRecord(c->ev1, c->stream));
-  std::vector<double> res((size_t)npairs * PS_OUT_STRIDE);
-  PS_HIP(c, hipMemcpyAsync(res.data(), c->d_out, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  PS_HIP(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
-  PS_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  c->last_ms = ms;
-  c->last_launches = 1;
-  if (std::getenv("PS_PHASE_TIMING")) {
-    unsigned long long cyc[16];
-    if (ps_phase_cycles(cyc, true) == hipSuccess) {
-      std::fprintf(stderr, "phase cycles (thread 0 of each workgroup, summed):");
-      for (int i = 0; i < 9; ++i) std::fprintf(stderr, " %d:%.4g", i, (double)cyc[i]);
-      std::fprintf(stderr, "\n");
-    }
-  }
-  for (int i = 0; i < npairs; ++i) {
-    const double* o = &res[(size_t)i * PS_OUT_STRIDE];
-    ps_pair_result& r = out[i];
-    r.dev = o[0];
-    r.dark = o[1];
-    r.period = o[2];
-    r.lambda_deg = o[3];
-    r.beta_deg = o[4];
-    r.chisq = o[5];
-    r.niter = (int)o[6];
-    r.reported = (int)o[7];
-    r.n_vis = (long long)o[8];
-    c->last_vis += r.n_vis;
-    c->last_iters += r.niter;
-    c->last_evals += r.niter > 0 ? r.niter + 1 : 0;
-  }
-  c->last_pairs = npairs;
-  return PS_OK;
+  std::vector<double> cr(npairs, conw_r * conw_r), cd(npairs, conw_r_dev);
+  return fit_items(c, npairs, nullptr, n_list, m_list, cr.data(), cd.data(), out);
+}
+
+int ps_fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list, const double* conw_r,
+                 const double* conw_r_dev, ps_pair_result* out) {
+  if (!c || n_items < 0 || (n_items && (!prob || !n_list || !m_list || !conw_r || !conw_r_dev || !out)))
+    return PS_E_ARG;
+  if (!c->prepared) return fail(c, PS_E_STATE, "ps_fit_items before ps_prepare");
+  std::vector<double> cr(n_items);
+  for (int i = 0; i < n_items; ++i) cr[i] = conw_r[i] * conw_r[i];
+  return fit_items(c, n_items, prob, n_list, m_list, cr.data(), conw_r_dev, out);
 }
 
 int ps_last_stats(const ps_ctx* c, double* kernel_ms, int* launches, long long* pairs, long long* vis_pairs,
@@ -487,27 +666,57 @@ int ps_precalc(ps_ctx* c, double* conw_r, int* new_conw) {
   // host pointer (start_CUDA.cu:476): a subnormal whose square is 0
   int rc = ps_fit_pairs(c, (int)nl.size(), nl.data(), ml.data(), *conw_r, 0.0, r.data());
   if (rc) return rc;
-  double sum_dark = 0.0;
-  for (int n = 0; n < tests; ++n) {
-    const int b = best_pole(&r[n * PS_N_POLES]);
-    if (b >= 0) sum_dark = sum_dark + r[n * PS_N_POLES + b].dark;
-  }
-  const double ave = sum_dark / tests;
-  if (ave < 1.0) *new_conw = 1;
-  if (ave >= 1.0) *conw_r = *conw_r * 2;
+  conw_decide(r.data(), conw_r, new_conw);
   return PS_OK;
 }
 
 int ps_conw_search(ps_ctx* c, double* conw_r) {
   if (!c || !conw_r) return PS_E_ARG;
-  if (!c->have_file) return fail(c, PS_E_STATE, "ps_conw_search needs ps_prepare_file");
-  double cw = c->conw / c->escl / c->escl;
+  if (!c->prepared || !c->hp[0].have_file) return fail(c, PS_E_STATE, "ps_conw_search needs ps_prepare_file");
+  const HostProb& h = c->hp[0];
+  double cw = h.conw / h.escl / h.escl;
   int new_conw = 0;
-  while (new_conw != 1 && (cw * c->escl * c->escl) < 10.0) {
+  while (new_conw != 1 && (cw * h.escl * h.escl) < 10.0) {
     int rc = ps_precalc(c, &cw, &new_conw);
     if (rc) return rc;
   }
   *conw_r = cw;
+  return PS_OK;
+}
+
+int ps_conw_search_batch(ps_ctx* c, double* conw_r) {
+  if (!c || !conw_r) return PS_E_ARG;
+  if (!c->prepared) return fail(c, PS_E_STATE, "ps_conw_search_batch before ps_prepare_files");
+  const int np = (int)c->hp.size();
+  for (int i = 0; i < np; ++i)
+    if (!c->hp[i].have_file) return fail(c, PS_E_STATE, "ps_conw_search_batch needs ps_prepare_files");
+  // every problem runs the loop of period_search_BOINC.cpp:647-771; the
+  // rounds of all undecided problems share one launch
+  std::vector<double> cw(np);
+  std::vector<int> done(np, 0);
+  for (int i = 0; i < np; ++i) cw[i] = c->hp[i].conw / c->hp[i].escl / c->hp[i].escl;
+  for (;;) {
+    std::vector<int> act;
+    for (int i = 0; i < np; ++i)
+      if (!done[i] && (cw[i] * c->hp[i].escl * c->hp[i].escl) < 10.0) act.push_back(i);
+    if (act.empty()) break;
+    std::vector<int> ip, nl, ml;
+    std::vector<double> cr, cd;
+    for (int i : act)
+      for (int n = 1; n <= 10; ++n)
+        for (int m = 1; m <= PS_N_POLES; ++m) {
+          ip.push_back(i);
+          nl.push_back(n);
+          ml.push_back(m);
+          cr.push_back(cw[i] * cw[i]);
+          cd.push_back(0.0);
+        }
+    std::vector<ps_pair_result> r(ip.size());
+    int rc = fit_items(c, (int)ip.size(), ip.data(), nl.data(), ml.data(), cr.data(), cd.data(), r.data());
+    if (rc) return rc;
+    for (size_t k = 0; k < act.size(); ++k) conw_decide(&r[k * 100], &cw[act[k]], &done[act[k]]);
+  }
+  for (int i = 0; i < np; ++i) conw_r[i] = cw[i];
   return PS_OK;
 }
 
@@ -523,21 +732,88 @@ int ps_start(ps_ctx* c, int n_start, int n_count, double conw_r, ps_line* out) {
   std::vector<ps_pair_result> r(nl.size());
   int rc = ps_fit_pairs(c, (int)nl.size(), nl.data(), ml.data(), conw_r, conw_r, r.data());
   if (rc) return rc;
-  for (int i = 0; i < n_count; ++i) {
-    ps_line& L = out[i];
-    const int b = best_pole(&r[(size_t)i * PS_N_POLES]);
-    if (b < 0) {
-      L = ps_line{NAN, NAN, NAN, NAN, NAN, NAN};
-      continue;
-    }
-    const ps_pair_result& x = r[(size_t)i * PS_N_POLES + b];
-    L.period_h = 24 * x.period;
-    L.rms = x.dev;
-    L.chi2 = x.dev * x.dev * c->ndata;
-    L.dark_or_conw = (n_start + i == 1) ? conw_r * c->escl * c->escl : x.dark;
-    L.lambda_deg = x.lambda_deg;
-    L.beta_deg = x.beta_deg;
+  make_lines(c->hp[0], n_start, n_count, conw_r, r.data(), out);
+  return PS_OK;
+}
+
+int ps_start_batch(ps_ctx* c, const int* n_start, const int* n_count, const double* conw_r, ps_line* out) {
+  if (!c || !n_start || !n_count || !conw_r) return PS_E_ARG;
+  if (!c->prepared) return fail(c, PS_E_STATE, "ps_start_batch before ps_prepare");
+  const int np = (int)c->hp.size();
+  std::vector<int> ip, nl, ml;
+  std::vector<double> cr, cd;
+  size_t lines = 0;
+  for (int i = 0; i < np; ++i) {
+    if (n_start[i] < 1 || n_count[i] < 0) return PS_E_ARG;
+    lines += n_count[i];
+    for (int n = n_start[i]; n < n_start[i] + n_count[i]; ++n)
+      for (int m = 1; m <= PS_N_POLES; ++m) {
+        ip.push_back(i);
+        nl.push_back(n);
+        ml.push_back(m);
+        cr.push_back(conw_r[i] * conw_r[i]);
+        cd.push_back(conw_r[i]);
+      }
   }
+  if (lines && !out) return PS_E_ARG;
+  if ((long long)ip.size() > 0x7fffffffLL) return fail(c, PS_E_ARG, "too many pairs in one call");
+  std::vector<ps_pair_result> r(ip.size());
+  int rc = fit_items(c, (int)ip.size(), ip.data(), nl.data(), ml.data(), cr.data(), cd.data(), r.data());
+  if (rc) return rc;
+  size_t li = 0, ri = 0;
+  for (int i = 0; i < np; ++i) {
+    make_lines(c->hp[i], n_start[i], n_count[i], conw_r[i], &r[ri], out + li);
+    li += n_count[i];
+    ri += (size_t)n_count[i] * PS_N_POLES;
+  }
+  return PS_OK;
+}
+
+int ps_set_launch(ps_ctx* c, int waves_per_pair, int wg_per_cu) {
+  if (!c || (waves_per_pair != 4 && waves_per_pair != 8) || wg_per_cu < 0) return PS_E_ARG;
+  c->waves = waves_per_pair;
+  c->wg_per_cu = wg_per_cu;
+  return PS_OK;
+}
+
+int ps_get_launch(const ps_ctx* c, int* waves_per_pair, int* wg_per_cu) {
+  if (!c || !c->prepared) return PS_E_STATE;
+  const bool w8 = c->waves == 8 && c->per_cu8 > 0;
+  const int per_cu = w8 ? c->per_cu8 : c->per_cu4;
+  if (waves_per_pair) *waves_per_pair = w8 ? 8 : 4;
+  if (wg_per_cu) *wg_per_cu = c->wg_per_cu > 0 ? std::min(c->wg_per_cu, per_cu) : per_cu;
+  return PS_OK;
+}
+
+int ps_autotune(ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list, const double* conw_r,
+                int* waves_per_pair, int* wg_per_cu, double* best_ms) {
+  if (!c || n_items <= 0 || !prob || !n_list || !m_list || !conw_r) return PS_E_ARG;
+  if (!c->prepared) return fail(c, PS_E_STATE, "ps_autotune before ps_prepare");
+  std::vector<ps_pair_result> r(n_items);
+  int bw = 4, bg = 0;
+  double bms = 0.0;
+  bool have = false;
+  for (int w : {4, 8}) {
+    const int per_cu = w == 8 ? c->per_cu8 : c->per_cu4;
+    if (per_cu < 1) continue;
+    for (int g = 1; g <= per_cu; ++g) {
+      c->waves = w;
+      c->wg_per_cu = g;
+      int rc = ps_fit_items(c, n_items, prob, n_list, m_list, conw_r, conw_r, r.data());
+      if (rc) return rc;
+      if (!have || c->last_ms < bms) {
+        have = true;
+        bms = c->last_ms;
+        bw = w;
+        bg = g;
+      }
+    }
+  }
+  c->waves = bw;
+  c->wg_per_cu = bg;
+  if (waves_per_pair) *waves_per_pair = bw;
+  if (wg_per_cu) *wg_per_cu = bg;
+  if (best_ms) *best_ms = bms;
   return PS_OK;
 }
 
@@ -561,11 +837,11 @@ int ps_eval(ps_ctx* c, const double* a, double conw_r2_rows, double* alpha, doub
   PS_HIP(c, hipMalloc(&dw, (size_t)K * K * sizeof(double)));
   PS_HIP(c, hipMalloc(&dar, c->nfac * sizeof(double)));
   PS_HIP(c, hipMemcpy(da, a, ma * sizeof(double), hipMemcpyHostToDevice));
-  PsFitArgs A = make_args(c, 0.0, 0.0, 1);
+  PsFitArgs A = make_args(c, 1);
   A.conw_r2_rows = conw_r2_rows;
   std::vector<double> w((size_t)K * K), ar(c->nfac);
   if (alpha || beta || chisq || area) {
-    PS_HIP(c, ps_launch_eval(A, c->stream, da, dw, dar, nullptr));
+    PS_HIP(c, psw4::ps_launch_eval(A, c->stream, da, dw, dar, nullptr));
     PS_HIP(c, hipStreamSynchronize(c->stream));
     PS_HIP(c, hipMemcpy(w.data(), dw, w.size() * sizeof(double), hipMemcpyDeviceToHost));
     PS_HIP(c, hipMemcpy(ar.data(), dar, ar.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -578,10 +854,11 @@ int ps_eval(ps_ctx* c, const double* a, double conw_r2_rows, double* alpha, doub
     if (area) std::memcpy(area, ar.data(), ar.size() * sizeof(double));
   }
   if (jrows) {
-    PS_HIP(c, hipMalloc(&dj, (size_t)(c->ndata + 3) * (ma + 1) * sizeof(double)));
-    PS_HIP(c, ps_launch_eval(A, c->stream, da, dw, dar, dj));
+    const size_t nj = (size_t)(c->hp[0].ndata + 3) * (ma + 1);
+    PS_HIP(c, hipMalloc(&dj, nj * sizeof(double)));
+    PS_HIP(c, psw4::ps_launch_eval(A, c->stream, da, dw, dar, dj));
     PS_HIP(c, hipStreamSynchronize(c->stream));
-    PS_HIP(c, hipMemcpy(jrows, dj, (size_t)(c->ndata + 3) * (ma + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    PS_HIP(c, hipMemcpy(jrows, dj, nj * sizeof(double), hipMemcpyDeviceToHost));
     hipFree(dj);
   }
   hipFree(da);
@@ -602,7 +879,7 @@ int ps_gaussj(ps_ctx* c, int mfit, const double* alpha, const double* beta, doub
   PS_HIP(c, hipMalloc(&de, sizeof(int)));
   PS_HIP(c, hipMemcpy(da, alpha, (size_t)mfit * mfit * sizeof(double), hipMemcpyHostToDevice));
   PS_HIP(c, hipMemcpy(db, beta, mfit * sizeof(double), hipMemcpyHostToDevice));
-  PS_HIP(c, ps_launch_gaussj(mfit, da, db, lambda, dx, de, c->stream));
+  PS_HIP(c, psw4::ps_launch_gaussj(mfit, da, db, lambda, dx, de, c->stream));
   PS_HIP(c, hipStreamSynchronize(c->stream));
   PS_HIP(c, hipMemcpy(x, dx, mfit * sizeof(double), hipMemcpyDeviceToHost));
   PS_HIP(c, hipMemcpy(err, de, sizeof(int), hipMemcpyDeviceToHost));
@@ -621,7 +898,7 @@ int ps_math_probe(ps_ctx* c, int n, const double* x, double* out, double* out2) 
   PS_HIP(c, hipMalloc(&d1, 3 * (size_t)n * sizeof(double)));
   PS_HIP(c, hipMalloc(&d2, n * sizeof(double)));
   PS_HIP(c, hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
-  PS_HIP(c, ps_launch_math(n, dx, d1, d2, c->stream));
+  PS_HIP(c, psw4::ps_launch_math(n, dx, d1, d2, c->stream));
   PS_HIP(c, hipStreamSynchronize(c->stream));
   PS_HIP(c, hipMemcpy(out, d1, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   PS_HIP(c, hipMemcpy(out2, d2, n * sizeof(double), hipMemcpyDeviceToHost));

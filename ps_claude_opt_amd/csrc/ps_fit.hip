@@ -34,7 +34,20 @@
 #include "ps_math.h"
 #include "ps_kernels.h"
 
+// This file is compiled once per workgroup size: PS_FIT_WAVES wavefronts per
+// workgroup, everything in namespace PS_FIT_NS.  The variants give
+// bit-identical results (the point chains of the normal equations do not
+// depend on the wave count) and differ only in speed.
+#ifndef PS_FIT_WAVES
+#define PS_FIT_WAVES 4
+#endif
+#ifndef PS_FIT_NS
+#define PS_FIT_NS psw4
+#endif
+
 typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace PS_FIT_NS {
 
 #ifdef PS_PHASE_TIMING
 // Debug builds only: cycles spent per phase by thread 0 of every workgroup
@@ -211,11 +224,11 @@ __device__ __forceinline__ void point_geometry(const PsDevProblem& P, const Inv&
 struct Lds {
   double* work;    // [K*K] normal matrix / augmented Gauss-Jordan matrix
   double* nrm;     // [F][4] facet normal x, y, z and the current facet area
-  double* ytab;    // [F][16][nbs] shape basis (LDS copy, or the global table)
+  const double* ytab;  // [F][16][nbs] shape basis (global table, read through L2)
   double* cgg;     // [ma]
   double* atry;    // [ma]
   double* rowbuf;  // [NW][16][16] per-point non-shape values
-  double* red;     // [8] reduction scratch
+  double* red;     // [2*NW] reduction scratch
   double* inv;     // [16] pair invariants of the evaluated parameters
 };
 
@@ -236,7 +249,7 @@ __device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
   return s;
 }
 
-__device__ Lds carve_lds(double* base, const PsDevProblem& P, bool ylds) {
+__device__ Lds carve_lds(double* base, const PsDevProblem& P) {
   Lds L;
   const int K = P.mfit + 1;
   double* p = base;
@@ -244,12 +257,7 @@ __device__ Lds carve_lds(double* base, const PsDevProblem& P, bool ylds) {
   p += K * K;
   L.nrm = p;
   p += 4 * P.nfac;
-  if (ylds) {
-    L.ytab = p;
-    p += (size_t)P.nfac * 16 * P.nbs;
-  } else {
-    L.ytab = const_cast<double*>(P.yl);
-  }
+  L.ytab = P.yl;
   L.cgg = p;
   p += P.ma;
   L.atry = p;
@@ -257,24 +265,19 @@ __device__ Lds carve_lds(double* base, const PsDevProblem& P, bool ylds) {
   L.rowbuf = p;
   p += NW * 256;
   L.red = p;
-  p += 8;
+  p += 2 * NW;
   L.inv = p;
   return L;
 }
 
-// Copies the facet normals (and the basis table when it lives in LDS) into
-// the workgroup's LDS once per launch.
-__device__ void stage_tables(const PsDevProblem& P, const Lds& L, bool ylds) {
+// Copies the facet normals into the workgroup's LDS once per launch.
+__device__ void stage_tables(const PsDevProblem& P, const Lds& L) {
   const int F = P.nfac;
   for (int f = threadIdx.x; f < F; f += WG) {
     L.nrm[4 * f + 0] = P.nx[f];
     L.nrm[4 * f + 1] = P.ny[f];
     L.nrm[4 * f + 2] = P.nz[f];
     L.nrm[4 * f + 3] = 0.0;
-  }
-  if (ylds) {
-    const int n = F * 16 * P.nbs;
-    for (int i = threadIdx.x; i < n; i += WG) L.ytab[i] = P.yl[i];
   }
   __syncthreads();
 }
@@ -878,183 +881,205 @@ struct PairState {
 // ---------------------------------------------------------------------------
 // Persistent LM fit kernel.
 // ---------------------------------------------------------------------------
-template <int NB, int NBS, bool YLDS>
+namespace {
+
+// The whole LM fit of work item `item` on problem P (Start.cu:4346-5248):
+// initialisation, up to n_iter_max iterations of mrqmin + Iter2, FinishPole.
+template <int NB, int NBS>
+__device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem& P, const Lds& L, PairState& S,
+                                         double* slab, int item) {
+  const int tid = threadIdx.x;
+  const int mf = A.P0.mfit, K = mf + 1, C = A.P0.ncoef, ma = A.P0.ma, F = A.P0.nfac;
+  const Slab SL = make_slab(P, slab);
+  double* best = SL.best;  // [K*K] accepted alpha | beta | chi^2
+  const double conw_r2_rows = A.item_cw[2 * item], conw_r_dev = A.item_cw[2 * item + 1];
+  const int n = A.item_n[item], m = A.item_m[item];  // 1-based
+  // pair initialisation (Start.cu:4346-4418)
+  if (tid == 0) {
+    const double period = 1.0 / (P.f_start - (double)(n - 1) * P.f_step);
+    const double d2r = kPi / 180.0;
+    L.cgg[C] = d2r * 90 - d2r * P.beta_pole[m - 1];
+    L.cgg[C + 1] = d2r * P.lambda_pole[m - 1];
+    L.cgg[C + 2] = (24.0 * 2.0 * kPi) / period;
+    L.cgg[C + 3] = P.par[0];
+    L.cgg[C + 4] = P.par[1];
+    L.cgg[C + 5] = P.par[2];
+    L.cgg[C + 6] = P.lcl;
+    L.cgg[C + 7] = 1;
+    S.alamda = -1;
+    S.niter = 0;
+    S.iter_diff = 1e40;
+    S.dev_old = 1e30;
+    S.dev_new = 0;
+    S.chck = 0;
+    S.chisq = 0;
+    S.ochisq = 0;
+    S.nvis = 0;
+  }
+  for (int i = tid; i < C; i += WG) L.cgg[i] = P.cg_first[i];
+  __syncthreads();
+  for (;;) {
+    // Iter1Begin (Start.cu:4421-4462)
+    const int niter = S.niter;
+    const bool is_niter = ((niter < P.n_iter_max) && (S.iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min);
+    if (!is_niter) break;
+    const bool is_alambda = S.alamda < 0;
+    __syncthreads();
+    if (is_alambda && tid == 0) S.alamda = P.alamda_start;
+    // pass 0 (first iteration only): normal equations at cgg;
+    // pass 1: damped solve (mrqmin step 1, Start.cu:2838-3021), then the
+    // normal equations at the trial point atry.
+#pragma unroll 1
+    for (int pass = is_alambda ? 0 : 1; pass < 2; ++pass) {
+      if (pass == 1) {
+        PS_TSTART();
+        // [alpha (diagonal * (1 + lambda)) | beta]
+        const double ccc = 1 + S.alamda;
+        for (int e = tid; e < mf * K; e += WG) {
+          const int i = e / K, j = e - i * K;
+          double vv;
+          if (j < mf) {
+            vv = best[i * K + j];
+            if (i == j) vv *= ccc;
+          } else {
+            vv = best[mf * K + i];
+          }
+          L.work[e] = vv;
+        }
+        __syncthreads();
+        const int err = gaussj_block<(NB > 4)>(L.work, mf, K, L.red);
+        __syncthreads();
+        if (err == 0 || err == 2)
+          for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.work[q * K + mf];
+        __syncthreads();
+        PS_TMARK(7);
+      }
+      evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, conw_r2_rows, SL, &S.nvis);
+      if (pass == 0) {
+        for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
+        if (tid == 0) S.ochisq = L.work[mf * K + mf];
+        for (int i = tid; i < ma; i += WG) L.atry[i] = L.cgg[i];
+        __syncthreads();
+      }
+    }
+    const double chisq_try = L.work[mf * K + mf];
+    // mrqmin step 2 (Start.cu:300-392)
+    const bool accept = chisq_try < S.ochisq;
+    if (accept) {
+      for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
+      for (int i = tid; i < ma; i += WG) L.cgg[i] = L.atry[i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (accept) {
+        S.alamda = S.alamda * P.alamda_incrr;
+        S.chisq = chisq_try;
+      } else {
+        S.alamda = P.alamda_incr * S.alamda;
+        S.chisq = S.ochisq;
+      }
+      S.niter = S.niter + 1;
+    }
+    __syncthreads();
+    // Iter2 (Start.cu:5129-5248)
+    PS_TSTART();
+    if (S.niter == 1 || S.chisq < S.ochisq) {
+      curv_block(P, L.cgg, L.nrm);
+      __syncthreads();
+      if (tid < 3) {
+        double s = 0.0;
+        for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3], L.nrm[4 * f + tid], s);
+        L.red[tid] = s;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        S.ochisq = S.chisq;
+        const double v0 = L.red[0], v1 = L.red[1], v2 = L.red[2];
+        const double aa = fma(v2, v2, fma(v1, v1, v0 * v0));
+        const double conwr2 = conw_r_dev * conw_r_dev;
+        const double rchisq = fma(-aa, conwr2, S.chisq);
+        S.dev_new = sqrt(rchisq / (double)P.ndata);
+        S.chck = sqrt(aa);
+        const double diff = S.dev_old - S.dev_new;
+        if (diff > 1e-10) {
+          S.iter_diff = diff;
+          S.dev_old = S.dev_new;
+        }
+      }
+    }
+    __syncthreads();
+    PS_TMARK(8);
+  }
+  // FinishPole (Start.cu:5056-5125)
+  if (tid == 0) {
+    double* o = A.out + (size_t)item * PS_OUT_STRIDE;
+    double dev = 1e40, dark = 0, per = 0, la = 0, be = 0;
+    if (!(S.dev_new >= 1e40)) {
+      double tot = 0.0;
+      for (int f = 0; f < F; ++f) tot = tot + L.nrm[4 * f + 3];
+      tot = 1.0 / tot;
+      double dd = (S.chck * 100.0) * tot;
+      if (dd != dd) dd = 1.0;
+      dev = S.dev_new;
+      dark = dd;
+      per = (2 * kPi) / L.cgg[C + 2];
+      const double r2d = 180.0 / kPi;
+      const double lt = r2d * L.cgg[C + 1];
+      la = lt + (lt < 0 ? 360 : 0);
+      be = 90 - r2d * L.cgg[C];
+    }
+    o[0] = dev;
+    o[1] = dark;
+    o[2] = per;
+    o[3] = la;
+    o[4] = be;
+    o[5] = S.chisq;
+    o[6] = (double)S.niter;
+    o[7] = 1.0;
+    o[8] = (double)S.nvis;
+  }
+  __syncthreads();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Persistent LM fit kernel: workgroups take work items from an atomic queue
+// until it is empty (BATCH: items of several problems, described in global
+// memory; otherwise every item is on problem P0, passed as an argument).
+// ---------------------------------------------------------------------------
+template <int NB, int NBS, bool BATCH>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void ps_fit_kernel(PsFitArgs A) {
   extern __shared__ double lds_dyn[];
   __shared__ PairState S;
   __shared__ int item_s;
-  const PsDevProblem& P = A.P;
-  const Lds L = carve_lds(lds_dyn, P, YLDS);
-  stage_tables(P, L, YLDS);
-  const int tid = threadIdx.x;
-  const int mf = P.mfit, K = mf + 1, C = P.ncoef, ma = P.ma, F = P.nfac;
-  const Slab SL = make_slab(P, A.scratch + (size_t)blockIdx.x * A.scratch_stride);
-  double* best = SL.best;  // [K*K] accepted alpha | beta | chi^2
+  // geometry and basis shared by every problem of the launch
+  const Lds L = carve_lds(lds_dyn, A.P0);
+  stage_tables(A.P0, L);
+  double* const slab = A.scratch + (size_t)blockIdx.x * A.scratch_stride;
   for (;;) {
-    if (tid == 0) item_s = atomicAdd(A.counter, 1);
+    if (threadIdx.x == 0) item_s = atomicAdd(A.counter, 1);
     __syncthreads();
     const int item = item_s;
     __syncthreads();
     if (item >= A.n_items) break;
-    const int n = A.item_n[item], m = A.item_m[item];  // 1-based
-    // pair initialisation (Start.cu:4346-4418)
-    if (tid == 0) {
-      const double period = 1.0 / (P.f_start - (double)(n - 1) * P.f_step);
-      const double d2r = kPi / 180.0;
-      L.cgg[C] = d2r * 90 - d2r * P.beta_pole[m - 1];
-      L.cgg[C + 1] = d2r * P.lambda_pole[m - 1];
-      L.cgg[C + 2] = (24.0 * 2.0 * kPi) / period;
-      L.cgg[C + 3] = P.par[0];
-      L.cgg[C + 4] = P.par[1];
-      L.cgg[C + 5] = P.par[2];
-      L.cgg[C + 6] = P.lcl;
-      L.cgg[C + 7] = 1;
-      S.alamda = -1;
-      S.niter = 0;
-      S.iter_diff = 1e40;
-      S.dev_old = 1e30;
-      S.dev_new = 0;
-      S.chck = 0;
-      S.chisq = 0;
-      S.ochisq = 0;
-      S.nvis = 0;
-    }
-    for (int i = tid; i < C; i += WG) L.cgg[i] = P.cg_first[i];
-    __syncthreads();
-    for (;;) {
-      // Iter1Begin (Start.cu:4421-4462)
-      const int niter = S.niter;
-      const bool is_niter = ((niter < P.n_iter_max) && (S.iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min);
-      if (!is_niter) break;
-      const bool is_alambda = S.alamda < 0;
-      __syncthreads();
-      if (is_alambda && tid == 0) S.alamda = P.alamda_start;
-      // pass 0 (first iteration only): normal equations at cgg;
-      // pass 1: damped solve (mrqmin step 1, Start.cu:2838-3021), then the
-      // normal equations at the trial point atry.
-#pragma unroll 1
-      for (int pass = is_alambda ? 0 : 1; pass < 2; ++pass) {
-        if (pass == 1) {
-          PS_TSTART();
-          // [alpha (diagonal * (1 + lambda)) | beta]
-          const double ccc = 1 + S.alamda;
-          for (int e = tid; e < mf * K; e += WG) {
-            const int i = e / K, j = e - i * K;
-            double vv;
-            if (j < mf) {
-              vv = best[i * K + j];
-              if (i == j) vv *= ccc;
-            } else {
-              vv = best[mf * K + i];
-            }
-            L.work[e] = vv;
-          }
-          __syncthreads();
-          const int err = gaussj_block<(NB > 4)>(L.work, mf, K, L.red);
-          __syncthreads();
-          if (err == 0 || err == 2)
-            for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.work[q * K + mf];
-          __syncthreads();
-          PS_TMARK(7);
-        }
-        evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, A.conw_r2_rows, SL, &S.nvis);
-        if (pass == 0) {
-          for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
-          if (tid == 0) S.ochisq = L.work[mf * K + mf];
-          for (int i = tid; i < ma; i += WG) L.atry[i] = L.cgg[i];
-          __syncthreads();
-        }
-      }
-      const double chisq_try = L.work[mf * K + mf];
-      // mrqmin step 2 (Start.cu:300-392)
-      const bool accept = chisq_try < S.ochisq;
-      if (accept) {
-        for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
-        for (int i = tid; i < ma; i += WG) L.cgg[i] = L.atry[i];
-      }
-      __syncthreads();
-      if (tid == 0) {
-        if (accept) {
-          S.alamda = S.alamda * P.alamda_incrr;
-          S.chisq = chisq_try;
-        } else {
-          S.alamda = P.alamda_incr * S.alamda;
-          S.chisq = S.ochisq;
-        }
-        S.niter = S.niter + 1;
-      }
-      __syncthreads();
-      // Iter2 (Start.cu:5129-5248)
-      PS_TSTART();
-      if (S.niter == 1 || S.chisq < S.ochisq) {
-        curv_block(P, L.cgg, L.nrm);
-        __syncthreads();
-        if (tid < 3) {
-          double s = 0.0;
-          for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3], L.nrm[4 * f + tid], s);
-          L.red[tid] = s;
-        }
-        __syncthreads();
-        if (tid == 0) {
-          S.ochisq = S.chisq;
-          const double v0 = L.red[0], v1 = L.red[1], v2 = L.red[2];
-          const double aa = fma(v2, v2, fma(v1, v1, v0 * v0));
-          const double conwr2 = A.conw_r_dev * A.conw_r_dev;
-          const double rchisq = fma(-aa, conwr2, S.chisq);
-          S.dev_new = sqrt(rchisq / (double)P.ndata);
-          S.chck = sqrt(aa);
-          const double diff = S.dev_old - S.dev_new;
-          if (diff > 1e-10) {
-            S.iter_diff = diff;
-            S.dev_old = S.dev_new;
-          }
-        }
-      }
-      __syncthreads();
-      PS_TMARK(8);
-    }
-    // FinishPole (Start.cu:5056-5125)
-    if (tid == 0) {
-      double* o = A.out + (size_t)item * PS_OUT_STRIDE;
-      double dev = 1e40, dark = 0, per = 0, la = 0, be = 0;
-      if (!(S.dev_new >= 1e40)) {
-        double tot = 0.0;
-        for (int f = 0; f < F; ++f) tot = tot + L.nrm[4 * f + 3];
-        tot = 1.0 / tot;
-        double dd = (S.chck * 100.0) * tot;
-        if (dd != dd) dd = 1.0;
-        dev = S.dev_new;
-        dark = dd;
-        per = (2 * kPi) / L.cgg[C + 2];
-        const double r2d = 180.0 / kPi;
-        const double lt = r2d * L.cgg[C + 1];
-        la = lt + (lt < 0 ? 360 : 0);
-        be = 90 - r2d * L.cgg[C];
-      }
-      o[0] = dev;
-      o[1] = dark;
-      o[2] = per;
-      o[3] = la;
-      o[4] = be;
-      o[5] = S.chisq;
-      o[6] = (double)S.niter;
-      o[7] = 1.0;
-      o[8] = (double)S.nvis;
-    }
-    __syncthreads();
+    if constexpr (BATCH)
+      fit_pair<NB, NBS>(A, A.probs[A.item_p[item]], L, S, slab, item);
+    else
+      fit_pair<NB, NBS>(A, A.P0, L, S, slab, item);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Single evaluation / Gauss-Jordan / math kernels for the parity tests.
 // ---------------------------------------------------------------------------
-template <int NB, int NBS, bool YLDS>
+template <int NB, int NBS>
 __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* a, double* out_work,
                                                      double* out_area, double* out_j) {
   extern __shared__ double lds_dyn[];
-  const PsDevProblem& P = A.P;
-  const Lds L = carve_lds(lds_dyn, P, YLDS);
-  stage_tables(P, L, YLDS);
+  const PsDevProblem& P = A.P0;
+  const Lds L = carve_lds(lds_dyn, P);
+  stage_tables(P, L);
   const int tid = threadIdx.x, K = P.mfit + 1;
   const Slab SL = make_slab(P, A.scratch);
   for (int i = tid; i < P.ma; i += WG) L.cgg[i] = a[i];
@@ -1120,24 +1145,21 @@ __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds) {
+size_t ps_fit_lds_bytes(const PsDevProblem& P) {
   const int K = P.mfit + 1;
-  size_t d = (size_t)K * K + 4 * P.nfac + 2 * P.ma + NW * 256 + 8 + 16;
-  if (ylds) d += (size_t)P.nfac * 16 * P.nbs;
+  const size_t d = (size_t)K * K + 4 * P.nfac + 2 * P.ma + NW * 256 + 2 * NW + 16;
   return d * sizeof(double) + 64;
 }
 
 // (NB, NBS) instantiations: NB = 16-column blocks of a Jacobian row, NBS =
-// blocks holding shape columns (NBS in {NB-1, NB}); YLDS = basis table in
-// LDS (when it fits next to the rest of the workgroup's LDS).
+// blocks holding shape columns (NBS in {NB-1, NB}).
 #define PS_FOR_EACH_SHAPE(X) X(1, 1) X(2, 1) X(2, 2) X(3, 2) X(3, 3) X(4, 3) X(4, 4) X(5, 4) X(5, 5) X(6, 5) X(6, 6)
 
 hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s) {
-  const bool yl = A.P.ylds != 0;
-  const size_t lds = ps_fit_lds_bytes(A.P, yl);
+  const size_t lds = ps_fit_lds_bytes(A.P0);
 #define PS_L(NB_, NBS_)                                                                                       \
-  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                     \
-    if (yl)                                                                                                   \
+  if (A.P0.nb == NB_ && A.P0.nbs == NBS_) {                                                                   \
+    if (A.batch)                                                                                              \
       hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_fit_kernel<NB_, NBS_, true>), dim3(grid), dim3(WG), lds, s, A);   \
     else                                                                                                      \
       hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_fit_kernel<NB_, NBS_, false>), dim3(grid), dim3(WG), lds, s, A);  \
@@ -1150,17 +1172,12 @@ hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s) {
 
 hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
                           double* jrows) {
-  const bool yl = A.P.ylds != 0;
-  const size_t lds = ps_fit_lds_bytes(A.P, yl);
-#define PS_L(NB_, NBS_)                                                                                       \
-  if (A.P.nb == NB_ && A.P.nbs == NBS_) {                                                                     \
-    if (yl)                                                                                                   \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_, true>), dim3(1), dim3(WG), lds, s, A, a,  \
-                         work, area, jrows);                                                                  \
-    else                                                                                                      \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_, false>), dim3(1), dim3(WG), lds, s, A, a, \
-                         work, area, jrows);                                                                  \
-    return hipGetLastError();                                                                                 \
+  const size_t lds = ps_fit_lds_bytes(A.P0);
+#define PS_L(NB_, NBS_)                                                                                        \
+  if (A.P0.nb == NB_ && A.P0.nbs == NBS_) {                                                                    \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(ps_eval_kernel<NB_, NBS_>), dim3(1), dim3(WG), lds, s, A, a, work, area, \
+                       jrows);                                                                                 \
+    return hipGetLastError();                                                                                  \
   }
   PS_FOR_EACH_SHAPE(PS_L)
 #undef PS_L
@@ -1195,32 +1212,37 @@ hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hip
   return hipGetLastError();
 }
 
-int ps_fit_blocks_per_cu(int nb, int nbs, bool ylds, size_t lds) {
+int ps_fit_blocks_per_cu(int nb, int nbs, size_t lds) {
 #define PS_O(NB_, NBS_)                                                                                       \
   if (nb == NB_ && nbs == NBS_) {                                                                             \
-    const void* fk = ylds ? reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, true>)                    \
-                          : reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, false>);                  \
-    int n = 0;                                                                                                \
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fk, WG, lds) != hipSuccess) return 0;                \
-    return n;                                                                                                 \
+    int n0 = 0, n1 = 0;                                                                                       \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, reinterpret_cast<const void*>(                      \
+                                                         &ps_fit_kernel<NB_, NBS_, false>), WG, lds) != hipSuccess || \
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, reinterpret_cast<const void*>(                      \
+                                                         &ps_fit_kernel<NB_, NBS_, true>), WG, lds) != hipSuccess)  \
+      return 0;                                                                                               \
+    return n0 < n1 ? n0 : n1;                                                                                 \
   }
   PS_FOR_EACH_SHAPE(PS_O)
 #undef PS_O
   return 0;
 }
 
-hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds) {
-#define PS_A(NB_, NBS_)                                                                                       \
-  if (nb == NB_ && nbs == NBS_) {                                                                             \
-    const void* fk = ylds ? reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, true>)                    \
-                          : reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, false>);                  \
-    const void* ek = ylds ? reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_, true>)                   \
-                          : reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_, false>);                 \
-    hipError_t e = hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);              \
-    if (e == hipSuccess) e = hipFuncSetAttribute(ek, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
-    return e;                                                                                                 \
+hipError_t ps_set_fit_attributes(int nb, int nbs, size_t lds) {
+#define PS_A(NB_, NBS_)                                                                                          \
+  if (nb == NB_ && nbs == NBS_) {                                                                                \
+    const void* k[3] = {reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, false>),                         \
+                        reinterpret_cast<const void*>(&ps_fit_kernel<NB_, NBS_, true>),                          \
+                        reinterpret_cast<const void*>(&ps_eval_kernel<NB_, NBS_>)};                              \
+    for (const void* f : k) {                                                                                    \
+      hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);               \
+      if (e != hipSuccess) return e;                                                                             \
+    }                                                                                                            \
+    return hipSuccess;                                                                                           \
   }
   PS_FOR_EACH_SHAPE(PS_A)
 #undef PS_A
   return hipErrorInvalidValue;
 }
+
+}  // namespace PS_FIT_NS

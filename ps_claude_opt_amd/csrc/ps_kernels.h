@@ -5,7 +5,6 @@
 
 #include <cstddef>
 
-#define PS_FIT_WAVES 4     // wavefronts per workgroup (one pair per workgroup)
 #define PS_OUT_STRIDE 10   // doubles per pair result
 
 // Device pointers and scalars of one prepared problem (passed by value).
@@ -28,7 +27,6 @@ struct PsDevProblem {
   const double *nx, *ny, *nz, *darea;
   const double *pleg, *fc, *fs;  // curv tables
   const double* yl;              // [F][16][nbs]: shape column q = 16b + r (parameter q+2) at [f][r][b]
-  int ylds;                      // 1: the fit kernel copies yl into LDS
   const double* cg_first;        // [ncoef]
   const int* cmap;               // [mfit] compact index -> parameter index (0-based)
   // model and LM controls
@@ -40,17 +38,25 @@ struct PsDevProblem {
   double beta_pole[10], lambda_pole[10];
 };
 
+// One persistent launch fits a list of work items; item i is pair
+// (item_n[i], item_m[i]) of problem item_p[i].  Every problem of a launch
+// shares the facet geometry and shape basis (nfac, lmax, mmax, the facet
+// tables, the free-parameter map); P0 carries those shared tables (it is
+// problem 0) and sizes the LDS.
 struct PsFitArgs {
-  PsDevProblem P;
-  double conw_r2_rows;  // sigr2 of the three convexity rows (conwR^2)
-  double conw_r_dev;    // conw_r used by Iter2's dev
+  PsDevProblem P0;
+  const PsDevProblem* probs;  // [n_problems] on the device
+  const int* item_p;    // [n_items] problem index
   const int* item_n;    // [n_items] frequency index, 1-based
   const int* item_m;    // [n_items] pole index, 1-based
+  const double* item_cw;  // [n_items][2] sigr2 of the convexity rows (conwR^2), conwR of Iter2's dev
   int n_items;
   int* counter;         // work-queue head, zeroed before launch
-  double* scratch;      // per-workgroup scratch slabs (ps_slab layout)
+  double* scratch;      // per-workgroup scratch slabs (ps_slab layout, sized for the largest problem)
   long long scratch_stride;
   double* out;          // [n_items][PS_OUT_STRIDE]
+  double conw_r2_rows;  // ps_eval only: sigr2 of the convexity rows
+  int batch;            // 1: items of several problems (read from probs), 0: every item is on P0
 };
 
 // Point chains of the normal-equation product: the waves of a workgroup
@@ -91,13 +97,21 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
   return s;
 }
 
-size_t ps_fit_lds_bytes(const PsDevProblem& P, bool ylds);
-hipError_t ps_set_fit_attributes(int nb, int nbs, bool ylds, size_t lds);
-int ps_fit_blocks_per_cu(int nb, int nbs, bool ylds, size_t lds);  // resident workgroups per CU
-hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s);
-hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work, double* area,
-                          double* jrows);
-hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda, double* x, int* err,
-                            hipStream_t s);
-hipError_t ps_phase_cycles(unsigned long long* out, bool reset);
-hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hipStream_t s);
+// Launch API of one compiled variant of the fit kernels (namespace = the
+// number of wavefronts per workgroup: psw4, psw8).
+#define PS_DECLARE_FIT_VARIANT(NS)                                                                  \
+  namespace NS {                                                                                    \
+  size_t ps_fit_lds_bytes(const PsDevProblem& P);                                                   \
+  hipError_t ps_set_fit_attributes(int nb, int nbs, size_t lds);                                    \
+  int ps_fit_blocks_per_cu(int nb, int nbs, size_t lds); /* resident workgroups per CU */           \
+  hipError_t ps_launch_fit(const PsFitArgs& A, int grid, hipStream_t s);                            \
+  hipError_t ps_launch_eval(const PsFitArgs& A, hipStream_t s, const double* a, double* work,       \
+                            double* area, double* jrows);                                           \
+  hipError_t ps_launch_gaussj(int mf, const double* alpha, const double* beta, double lambda,       \
+                              double* x, int* err, hipStream_t s);                                  \
+  hipError_t ps_phase_cycles(unsigned long long* out, bool reset);                                  \
+  hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hipStream_t s);      \
+  }
+
+PS_DECLARE_FIT_VARIANT(psw4)
+PS_DECLARE_FIT_VARIANT(psw8)

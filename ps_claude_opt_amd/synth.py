@@ -277,6 +277,45 @@ def config_seed(config: int, index: int = 0) -> int:
     return SEED_BASE + 1000 * config + index
 
 
+def batch_configs(n: int = 64, seed: Optional[int] = None) -> List[SynthConfig]:
+    """The batched ragged workload: n asteroids of the same generator with
+    point counts log-uniform in 300-8000 (dense : sparse = 1 : 2, dense
+    curves of up to 150 points, sparse curves of 100) and 1000-4000 trial
+    frequencies each, centred on the true period."""
+    rng = np.random.default_rng(config_seed(3) if seed is None else seed)
+    out = []
+    for _ in range(n):
+        npts = int(round(math.exp(rng.uniform(math.log(300.0), math.log(8000.0)))))
+        dense = npts // 3
+        n_dense = max(1, -(-dense // 150))
+        pts_dense = max(10, dense // n_dense)
+        n_sparse = max(10, npts - n_dense * pts_dense)
+        n_freq = int(rng.integers(1000, 4001))
+        out.append(dataclasses.replace(PRESETS["c1"], n_dense=n_dense, pts_dense=pts_dense, n_sparse=n_sparse,
+                                       sparse_group=100, n_freq=n_freq))
+    return out
+
+
+def write_batch(directory: str, n: int = 64, seed: Optional[int] = None,
+                overrides: Optional[dict] = None) -> List[dict]:
+    """Writes asteroid_000.in ... for the batched workload; returns the truth
+    dicts (with "path").  Asteroid i uses seed config_seed(3, i)."""
+    import os
+
+    os.makedirs(directory, exist_ok=True)
+    truths = []
+    for i, cfg in enumerate(batch_configs(n, seed)):
+        if overrides:
+            cfg = dataclasses.replace(cfg, **overrides)
+        text, truth = make_asteroid(cfg, config_seed(3, i))
+        path = os.path.join(directory, f"asteroid_{i:03d}.in")
+        with open(path, "w") as f:
+            f.write(text)
+        truth.update(path=path, preset="c3", index=i)
+        truths.append(truth)
+    return truths
+
+
 def write_asteroid(path: str, preset: str = "c1", seed: Optional[int] = None,
                    overrides: Optional[dict] = None) -> dict:
     cfg = dataclasses.replace(PRESETS[preset], **(overrides or {}))
