@@ -35,6 +35,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # HBM bytes per pair-fit of the fit kernel, from FETCH_SIZE / WRITE_SIZE
 # counter passes scaled by a plain copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ps_fit_kernel_traffic.json")
+WARM_WEIGHTS = os.path.join(ROOT, "ps_claude_opt_amd", "data", "warm_lmax6.npz")
 
 
 def measured_traffic(preset: str, pairs_per_launch: float):
@@ -68,6 +69,8 @@ def main() -> int:
     ap.add_argument("--conw-r", type=float, default=0.0, help="use this conwR instead of searching for it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-configuration autotuner")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
+    ap.add_argument("--warm-iters", type=int, default=10, help="LM iterations per pair of the warm-started scan")
     ap.add_argument("--work-dir", default=os.path.join(ROOT, "gpurun_out", "bench"))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates HBM byte counters)")
@@ -180,11 +183,45 @@ def main() -> int:
                       "beta": best.beta_deg, "truth_period_h": truth["period_h"],
                       "truth_lambda": truth["lambda_deg"], "truth_beta": truth["beta_deg"]},
     }
+    if not args.no_warm and n_max == info["n_max"] and os.path.exists(WARM_WEIGHTS):
+        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters)
     scan.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(path, conw_r, n_max, args.cpu_pairs)
     print(json.dumps(result), flush=True)
     return 0
+
+
+def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int) -> dict:
+    """The learned warm start on the same scan: every pair runs n_iter LM
+    iterations from the network's prediction, then the 10 best frequencies
+    are refitted cold (the plain scan's own 50-iteration fits), which decide
+    the reported winner.  The same truncated scan from the reference's start
+    is timed beside it (what the network adds), and both winners are compared
+    with the plain scan's (the same printed lambda and beta, period within
+    1e-5 relative, rms within 5e-4, chi2 within 0.5)."""
+    import numpy as np
+
+    from ps_claude_opt_amd import warm
+
+    ws = warm.WarmStart.load(WARM_WEIGHTS)
+    net = ws.network(scan)
+    out = {"n_iter": n_iter, "refit_frequencies": 10, "weights": os.path.relpath(WARM_WEIGHTS, ROOT)}
+    for name, w in (("warm", ws), ("truncated_cold", None)):
+        t0 = time.perf_counter()
+        res, refit = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=10)
+        dt = time.perf_counter() - t0
+        b = warm.best_line(res, 1, n_max)
+        b["chi2"] = b["rms"] * b["rms"] * scan.info["ndata"]  # the output line's chi2 (start_CUDA.cu:1222-1226)
+        agree = (round(b["lambda_deg"]) == round(plain_best.lambda_deg) and round(b["beta_deg"]) == round(plain_best.beta_deg)
+                 and abs(b["period_h"] - plain_best.period_h) <= 1e-5 * plain_best.period_h
+                 and abs(b["rms"] - plain_best.rms) <= 5e-4 and abs(b["chi2"] - plain_best.chi2) <= 0.5)
+        out[name] = {"value": round(n_max * 10 / dt, 3), "unit": "pair-fits/s", "seconds": round(dt, 3),
+                     "best_line": b, "agrees_with_plain": bool(agree)}
+        print(f"# {name}: {n_max * 10} pairs in {dt:.2f} s, best {b['period_h']:.6f} h rms {b['rms']:.6f} "
+              f"agrees {agree}", file=sys.stderr, flush=True)
+    net.close()
+    return out
 
 
 def autotune(scan, probs, n_max, conw, skip: bool) -> dict:

@@ -190,6 +190,69 @@ int ps_conw_search_batch(ps_ctx* ctx, double* conw_r /* [n] */);
  * i with conw_r[i]; the lines of problem i follow those of problems < i. */
 int ps_start_batch(ps_ctx* ctx, const int* n_start, const int* n_count, const double* conw_r, ps_line* out);
 
+/* ---- learned warm start (no counterpart in the reference; for callers
+ * that start each LM fit from a predicted shape instead of the ellipsoid of
+ * ellfit.c) ----
+ *
+ * Features of work item (problem p, frequency n, pole m), PS_WARM_FEATURES
+ * floats:
+ *   [0, 512)   up to 8 lightcurves (the first curves of the problem spanning
+ *              at most 2 days with at least 16 points) folded at the trial
+ *              period (phase = frac(t[days] * 24 / P[h])) into 64 bins: mean
+ *              of brightness / curve mean - 1 per bin, 0 for empty bins;
+ *   512..514   unit vector of the initial pole m (ecliptic);
+ *   515..520   mean asteroid->Earth and asteroid->Sun unit vectors of those
+ *              curves' points;
+ *   521        their mean solar phase angle (rad);
+ *   522        log(P[h] / 10);  523..527 zero. */
+#define PS_WARM_CURVES 8
+#define PS_WARM_BINS 64
+#define PS_WARM_FEATURES 528
+/* The reference's initial parameter vector of each item (Start.cu:
+ * 4346-4418), a0[k*ma ...], and the free-parameter mask (ia[i] = parameter
+ * i+1 is fitted) shared by the problems of the context. */
+int ps_initial_params(const ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
+                      double* a0);
+int ps_free_params(const ps_ctx* ctx, int* ia /* [ma] */);
+int ps_warm_n_features(void);
+int ps_warm_features(const ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
+                     float* out /* [n_items][PS_WARM_FEATURES] */);
+/* ps_fit_items starting from caller-given parameters: a0[k*ma + i] for the
+ * shape coefficients (i < ncoef) and the phase-function / Lambert
+ * parameters (i = ncoef+3 .. ncoef+6); the pole, rotation rate and the
+ * Lommel-Seeliger weight are set from (n, m) as in the reference
+ * (Start.cu:4346-4418).  a0 may be NULL (the reference's initialisation).
+ * n_iter > 0 runs exactly n_iter LM iterations (the reference's
+ * stopCondition > 1 mode) instead of each problem's own stop condition.
+ * a_final (may be NULL) receives every item's final parameter vector. */
+int ps_fit_items_from(ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
+                      const double* conw_r, const double* conw_r_dev, const double* a0, int n_iter,
+                      ps_pair_result* out, double* a_final);
+
+/* The warm-start network: n_layers hidden layers of n_hidden GELU units
+ * and a linear output of n_out, trained with BF16 operands and FP32
+ * accumulation (MFMA) and Adam, evaluated in FP32.  Parameters are FP32 in
+ * the order W0 [n_in][n_hidden], b0, W1, b1, ..., W_out [n_hidden][n_out],
+ * b_out (row-major).  Inputs and targets are used as given (standardise
+ * them first). */
+typedef struct ps_mlp ps_mlp;
+int ps_mlp_create(ps_ctx* ctx, int n_in, int n_hidden, int n_layers, int n_out, unsigned long long seed,
+                  ps_mlp** out);
+long long ps_mlp_n_params(const ps_mlp* net);
+int ps_mlp_get_params(const ps_mlp* net, float* params);
+int ps_mlp_set_params(ps_mlp* net, const float* params);
+/* y[rows][n_out] = net(x[rows][n_in]); bf16 = 1 evaluates with the training
+ * precision (BF16 operands), 0 in FP32. */
+int ps_mlp_forward(ps_mlp* net, int rows, const float* x, float* y, int bf16);
+/* Mean squared error over rows x n_out and its gradient with respect to
+ * every parameter (training precision), grads in ps_mlp_get_params order. */
+int ps_mlp_loss_grad(ps_mlp* net, int rows, const float* x, const float* t, float* loss, float* grads);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `epochs` passes of seeded shuffled
+ * minibatches; loss[e] = mean training loss of epoch e (may be NULL). */
+int ps_mlp_train(ps_mlp* net, int n, const float* x, const float* t, int epochs, int batch, float lr,
+                 unsigned long long seed, float* loss);
+int ps_mlp_free(ps_mlp* net);
+
 /* ---- launch configuration (no counterpart in the reference, whose kernel
  * launch shapes are compile-time constants; for callers tuning throughput
  * on their own workloads) ----

@@ -1002,7 +1002,8 @@ struct FitOut {
 };
 
 void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r_dev, pso_order ord,
-              pso_pair_result* res, double* traj, int traj_cap) {
+              pso_pair_result* res, double* traj, int traj_cap, const double* a0 = nullptr,
+              double* a_out = nullptr, int n_iter_fixed = 0) {
   const int ma = P.ma, mf = P.mfit, C = P.ncoef, nf = P.nfac;
   std::vector<double> cgg(ma), atry(ma), alpha(mf * mf), beta(mf), covar(mf * mf), da(mf), area(nf);
   /* Start.cu:4346-4418 */
@@ -1014,13 +1015,23 @@ void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r
   for (int i = 1; i <= 3; i++) cgg[C + 2 + i] = P.par[i];
   cgg[C + 6] = P.lcl;
   cgg[C + 7] = 1;
+  if (a0) {
+    /* caller-given start: shape and phase-function / Lambert parameters;
+       pole, rotation rate and cls as above */
+    for (int i = 0; i < C; i++) cgg[i] = a0[i];
+    for (int i = C + 3; i <= C + 6; i++) cgg[i] = a0[i];
+  }
   double Alamda = -1, iter_diff = 1e40, dev_old = 1e30, dev_new = 0, chck = 0;
   double Chisq = 0, Ochisq = 0;
   int Niter = 0;
   long long nevals = 0, nvis = 0;
   for (;;) {
     /* Iter1Begin */
-    bool isNiter = ((Niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (Niter < P.n_iter_min);
+    /* n_iter_fixed > 0: exactly that many iterations (stopCondition > 1 mode) */
+    const int it_max = n_iter_fixed > 0 ? n_iter_fixed : P.n_iter_max;
+    const double diff_max = n_iter_fixed > 0 ? 0.0 : P.iter_diff_max;
+    const int it_min = n_iter_fixed > 0 ? 0 : P.n_iter_min;
+    bool isNiter = ((Niter < it_max) && (iter_diff > diff_max)) || (Niter < it_min);
     if (!isNiter) break;
     bool isAlambda = false;
     if (Alamda < 0) { isAlambda = true; Alamda = P.alamda_start; }
@@ -1109,6 +1120,8 @@ void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r
     r.beta_deg = 90 - RAD2DEG * cgg[C];
   }
   *res = r;
+  if (a_out)
+    for (int i = 0; i < ma; i++) a_out[i] = cgg[i];
 }
 
 /* start_CUDA.cu:1200-1210: earliest reported pole with minimal dev, NaN-safe */
@@ -1120,7 +1133,8 @@ int best_pole(const pso_pair_result* r /* N_POLES */) {
 }
 
 double fit_many(const Problem& P, int npairs, const int* nl, const int* ml, double conw_r, double conw_r_dev,
-                pso_order ord, int threads, pso_pair_result* res) {
+                pso_order ord, int threads, pso_pair_result* res, const double* a0 = nullptr,
+                double* a_out = nullptr, int n_iter_fixed = 0) {
   if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
   if (threads < 1) threads = 1;
   if (threads > npairs) threads = npairs > 0 ? npairs : 1;
@@ -1131,7 +1145,8 @@ double fit_many(const Problem& P, int npairs, const int* nl, const int* ml, doub
     for (;;) {
       int i = next.fetch_add(1);
       if (i >= npairs) break;
-      fit_pair(P, nl[i], ml[i], cr2, conw_r_dev, ord, &res[i], nullptr, 0);
+      fit_pair(P, nl[i], ml[i], cr2, conw_r_dev, ord, &res[i], nullptr, 0, a0 ? a0 + (size_t)i * P.ma : nullptr,
+               a_out ? a_out + (size_t)i * P.ma : nullptr, n_iter_fixed);
     }
   };
   std::vector<std::thread> th;
@@ -1299,6 +1314,12 @@ int pso_conw_search(pso_problem* h, int threads, pso_order order, double* conw_r
 double pso_fit_pairs(const pso_problem* h, int npairs, const int* n_list, const int* m_list, double conw_r,
                      double conw_r_dev, pso_order order, int threads, pso_pair_result* res) {
   return fit_many(h->P, npairs, n_list, m_list, conw_r, conw_r_dev, order, threads, res);
+}
+
+double pso_fit_pairs_from(const pso_problem* h, int npairs, const int* n_list, const int* m_list, double conw_r,
+                          double conw_r_dev, int threads, const double* a0, int n_iter, pso_pair_result* res,
+                          double* a_out) {
+  return fit_many(h->P, npairs, n_list, m_list, conw_r, conw_r_dev, pso_order{4, 16}, threads, res, a0, a_out, n_iter);
 }
 
 void pso_set_stop(pso_problem* h, double stop_condition, int n_iter_min) { set_stop(h->P, stop_condition, n_iter_min); }

@@ -104,6 +104,14 @@ double pso_fit_pairs(const pso_problem* p, int npairs, const int* n_list,
                      const int* m_list, double conw_r, double conw_r_dev,
                      pso_order order, int threads, pso_pair_result* res);
 
+/* pso_fit_pairs starting every pair from a0[k*ma ...] (shape and phase-
+ * function / Lambert parameters; pole, rotation rate and cls from (n, m)),
+ * a0 may be NULL; n_iter > 0 runs exactly n_iter iterations; the final
+ * parameter vectors into a_out (may be NULL). */
+double pso_fit_pairs_from(const pso_problem* p, int npairs, const int* n_list, const int* m_list,
+                          double conw_r, double conw_r_dev, int threads, const double* a0, int n_iter,
+                          pso_pair_result* res, double* a_out);
+
 /* Test hook: override the LM iteration controls (n_iter_max, n_iter_min,
  * iter_diff_max) as if read from stopCondition / n_iter_min. */
 void pso_set_stop(pso_problem* p, double stop_condition, int n_iter_min);

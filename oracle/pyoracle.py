@@ -63,6 +63,9 @@ def load():
     L.pso_fit_pairs.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_double, C.c_double, Order, C.c_int,
                                 C.POINTER(PairResult)]
     L.pso_set_stop.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    L.pso_fit_pairs_from.restype = C.c_double
+    L.pso_fit_pairs_from.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_double, C.c_double, C.c_int, _dp,
+                                     C.c_int, C.POINTER(PairResult), _dp]
     _lib = L
     return L
 
@@ -144,6 +147,20 @@ class Problem:
         wall = self.L.pso_fit_pairs(self.h, len(n_arr), n_arr.ctypes.data_as(_ip), m_arr.ctypes.data_as(_ip),
                                     conw_r, conw_r_dev, order, threads, out)
         return list(out[:len(n_arr)]), wall
+
+    def fit_pairs_from(self, n_list, m_list, conw_r, conw_r_dev, a0=None, threads=0, want_final=False, n_iter=0):
+        """fit_pairs from given initial parameters a0 [pairs][ma] (None: the
+        reference's initialisation); returns (results, final params or None)."""
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        k = len(n_arr)
+        out = (PairResult * max(k, 1))()
+        a0p = None if a0 is None else np.ascontiguousarray(a0, np.float64)
+        af = np.zeros((k, self.info["ma"])) if want_final else None
+        self.L.pso_fit_pairs_from(self.h, k, n_arr.ctypes.data_as(_ip), m_arr.ctypes.data_as(_ip), conw_r,
+                                  conw_r_dev, threads, a0p.ctypes.data_as(_dp) if a0p is not None else None, n_iter, out,
+                                  af.ctypes.data_as(_dp) if want_final else None)
+        return list(out[:k]), af
 
     def conw_search(self, threads=0, order=DEFAULT_ORDER) -> float:
         cw = C.c_double(0)

@@ -193,6 +193,52 @@ class Scan:
                                            _abi.iptr(m_arr), _abi.dptr(cw), _abi.dptr(cd), out))
         return _results(out, len(p_arr))
 
+    # -- warm starts -----------------------------------------------------------
+    def warm_features(self, prob, n_list, m_list) -> np.ndarray:
+        p_arr = np.ascontiguousarray(prob, np.int32)
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        nf = self._lib.ps_warm_n_features()
+        out = np.zeros((len(p_arr), nf), np.float32)
+        self._check(self._lib.ps_warm_features(self._h, len(p_arr), _abi.iptr(p_arr), _abi.iptr(n_arr),
+                                               _abi.iptr(m_arr), _abi.fptr(out)))
+        return out
+
+    def initial_params(self, prob, n_list, m_list) -> np.ndarray:
+        """The reference's initial parameter vectors [items][ma]."""
+        p_arr = np.ascontiguousarray(prob, np.int32)
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        out = np.zeros((len(p_arr), self.info["ma"]))
+        self._check(self._lib.ps_initial_params(self._h, len(p_arr), _abi.iptr(p_arr), _abi.iptr(n_arr),
+                                                _abi.iptr(m_arr), _abi.dptr(out)))
+        return out
+
+    def free_params(self) -> np.ndarray:
+        ia = np.zeros(self.info["ma"], np.int32)
+        self._check(self._lib.ps_free_params(self._h, _abi.iptr(ia)))
+        return ia
+
+    def fit_items_from(self, prob, n_list, m_list, conw_r, conw_r_dev, a0=None, want_final: bool = False,
+                       n_iter: int = 0):
+        """ps_fit_items_from: fits starting from a0 [items][ma] (None: the
+        reference's initialisation), n_iter > 0 = exactly that many LM
+        iterations; returns (results, final parameters or None)."""
+        p_arr = np.ascontiguousarray(prob, np.int32)
+        n_arr = np.ascontiguousarray(n_list, np.int32)
+        m_arr = np.ascontiguousarray(m_list, np.int32)
+        k = len(p_arr)
+        cw = np.ascontiguousarray(np.broadcast_to(conw_r, (k,)), np.float64)
+        cd = np.ascontiguousarray(np.broadcast_to(conw_r_dev, (k,)), np.float64)
+        a0p = None if a0 is None else np.ascontiguousarray(a0, np.float64)
+        af = np.zeros((k, self.info["ma"])) if want_final else None
+        out = (_abi.PsPairResult * max(k, 1))()
+        self._check(self._lib.ps_fit_items_from(self._h, k, _abi.iptr(p_arr), _abi.iptr(n_arr), _abi.iptr(m_arr),
+                                                _abi.dptr(cw), _abi.dptr(cd),
+                                                _abi.dptr(a0p) if a0p is not None else None, n_iter, out,
+                                                _abi.dptr(af) if want_final else None))
+        return _results(out, k), af
+
     # -- launch configuration ------------------------------------------------
     def set_launch(self, waves_per_pair: int = 4, wg_per_cu: int = 0):
         self._check(self._lib.ps_set_launch(self._h, waves_per_pair, wg_per_cu))

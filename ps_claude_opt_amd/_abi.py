@@ -84,6 +84,23 @@ SIGNATURES = {
     "ps_fit_items": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, C.POINTER(PsPairResult)]),
     "ps_conw_search_batch": (C.c_int, [C.c_void_p, _dp]),
     "ps_start_batch": (C.c_int, [C.c_void_p, _ip, _ip, _dp, C.POINTER(PsLine)]),
+    "ps_initial_params": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp]),
+    "ps_free_params": (C.c_int, [C.c_void_p, _ip]),
+    "ps_warm_n_features": (C.c_int, []),
+    "ps_warm_features": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, C.POINTER(C.c_float)]),
+    "ps_fit_items_from": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, C.c_int,
+                                    C.POINTER(PsPairResult), _dp]),
+    "ps_mlp_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
+                                C.POINTER(C.c_void_p)]),
+    "ps_mlp_n_params": (C.c_longlong, [C.c_void_p]),
+    "ps_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "ps_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "ps_mlp_forward": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]),
+    "ps_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "ps_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.c_int,
+                               C.c_float, C.c_ulonglong, C.POINTER(C.c_float)]),
+    "ps_mlp_free": (C.c_int, [C.c_void_p]),
     "ps_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "ps_get_launch": (C.c_int, [C.c_void_p, _ip, _ip]),
     "ps_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp]),
@@ -111,6 +128,11 @@ def load(path: str = LIB_PATH):
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def iptr(a: np.ndarray):

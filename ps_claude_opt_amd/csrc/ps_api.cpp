@@ -26,52 +26,7 @@ namespace {
 constexpr int kMaxIter = 60;  // constants.h:12
 }
 
-struct ps_input {
-  ps::InputFile f;
-};
-
-// Host-side facts of one prepared problem.
-struct HostProb {
-  int ndata = 0, ncurves = 0, n_max = 0, n_iter_max = 0;
-  double f_start = 0, f_end = 0, f_step = 0, escl = 0, conw = 0;
-  bool have_file = false;
-};
-
-struct ps_ctx {
-  int device = 0;
-  int n_cu = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::string err;
-  // problems (one, or a batch sharing the shape basis)
-  bool prepared = false;
-  std::vector<HostProb> hp;
-  std::vector<PsDevProblem> dps;  // host copies of the device descriptors
-  PsDevProblem* d_probs = nullptr;
-  int nfac = 0, ncoef = 0, ma = 0, mfit = 0;
-  std::vector<void*> dbufs;
-  long long stride = 0;
-  // launch configuration: wavefronts per workgroup (one pair per workgroup)
-  // and resident workgroups per CU (0 = as many as fit)
-  int waves = 4;
-  int wg_per_cu = 0;
-  size_t lds4 = 0, lds8 = 0;
-  int per_cu4 = 0, per_cu8 = 0;
-  // work-list buffers
-  double* scratch = nullptr;
-  size_t scratch_elems = 0;
-  int* counter = nullptr;
-  int* d_items_p = nullptr;
-  int* d_items_n = nullptr;
-  int* d_items_m = nullptr;
-  double* d_items_cw = nullptr;
-  double* d_out = nullptr;
-  size_t items_cap = 0;
-  // statistics of the last fit call
-  double last_ms = 0;
-  int last_launches = 0;
-  long long last_pairs = 0, last_vis = 0, last_iters = 0, last_evals = 0;
-};
+#include "ps_ctx.h"
 
 namespace {
 
@@ -339,6 +294,19 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
     h.escl = p->escl;
     h.conw = conw ? conw[pi] : 0.0;
     h.have_file = have_file ? have_file[pi] : false;
+    h.l_points.assign(p->l_points, p->l_points + L);
+    h.tim.assign(p->tim, p->tim + N);
+    h.bright.assign(p->brightness, p->brightness + N);
+    h.ee.assign(p->ee, p->ee + 3 * (size_t)N);
+    h.ee0.assign(p->ee0, p->ee0 + 3 * (size_t)N);
+    for (int i = 0; i < PS_N_POLES; ++i) {
+      h.beta_pole[i] = p->beta_pole[i];
+      h.lambda_pole[i] = p->lambda_pole[i];
+    }
+    h.cg_first.assign(p->cg_first, p->cg_first + C);
+    for (int i = 0; i < 3; ++i) h.par[i] = p->par[i];
+    h.lcl = d.lcl;
+    h.phi0 = p->phi0;
     c->dps.push_back(d);
     c->hp.push_back(h);
     c->stride = std::max(c->stride, (long long)ps_slab(d).stride);
@@ -355,6 +323,7 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
   c->ncoef = C;
   c->ma = ma;
   c->mfit = mfit;
+  c->ia.assign(p0->ia, p0->ia + ma);
   const size_t lds_cap = 160 * 1024;
   c->lds4 = psw4::ps_fit_lds_bytes(c->dps[0]);
   c->lds8 = psw8::ps_fit_lds_bytes(c->dps[0]);
@@ -373,7 +342,8 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
 // largest problem first (the work of a pair grows with the problem's point
 // count), results come back in the caller's order.
 int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list, const double* cw_rows,
-              const double* cw_dev, ps_pair_result* out) {
+              const double* cw_dev, ps_pair_result* out, const double* a0 = nullptr, double* a_final = nullptr,
+              int n_iter_fixed = 0) {
   c->last_ms = 0;
   c->last_launches = 0;
   c->last_pairs = c->last_vis = c->last_iters = c->last_evals = 0;
@@ -410,15 +380,43 @@ int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const 
   PS_HIP(c, hipMemcpyAsync(c->d_items_n, in.data(), n_items * sizeof(int), hipMemcpyHostToDevice, c->stream));
   PS_HIP(c, hipMemcpyAsync(c->d_items_m, im.data(), n_items * sizeof(int), hipMemcpyHostToDevice, c->stream));
   PS_HIP(c, hipMemcpyAsync(c->d_items_cw, icw.data(), icw.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int ma = c->ma;
+  if ((a0 || a_final) && (size_t)n_items > c->a_cap) {
+    if (c->d_a0) hipFree(c->d_a0);
+    if (c->d_afinal) hipFree(c->d_afinal);
+    c->d_a0 = c->d_afinal = nullptr;
+    c->a_cap = 0;
+    PS_HIP(c, hipMalloc(&c->d_a0, (size_t)n_items * ma * sizeof(double)));
+    PS_HIP(c, hipMalloc(&c->d_afinal, (size_t)n_items * ma * sizeof(double)));
+    c->a_cap = n_items;
+  }
+  std::vector<double> a0p;
+  if (a0) {
+    a0p.resize((size_t)n_items * ma);
+    for (int k = 0; k < n_items; ++k)
+      std::memcpy(&a0p[(size_t)k * ma], a0 + (size_t)order[k] * ma, ma * sizeof(double));
+    PS_HIP(c, hipMemcpyAsync(c->d_a0, a0p.data(), a0p.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
   const int zero = 0;
   PS_HIP(c, hipMemcpyAsync(c->counter, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
   PsFitArgs A = make_args(c, n_items);
+  A.item_a0 = a0 ? c->d_a0 : nullptr;
+  A.out_a = a_final ? c->d_afinal : nullptr;
+  A.n_iter_fixed = n_iter_fixed;
   PS_HIP(c, hipEventRecord(c->ev0, c->stream));
   PS_HIP(c, w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream));
   PS_HIP(c, hipEventRecord(c->ev1, c->stream));
   std::vector<double> res((size_t)n_items * PS_OUT_STRIDE);
   PS_HIP(c, hipMemcpyAsync(res.data(), c->d_out, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> afp;
+  if (a_final) {
+    afp.resize((size_t)n_items * ma);
+    PS_HIP(c, hipMemcpyAsync(afp.data(), c->d_afinal, afp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
   PS_HIP(c, hipStreamSynchronize(c->stream));
+  if (a_final)
+    for (int k = 0; k < n_items; ++k)
+      std::memcpy(a_final + (size_t)order[k] * ma, &afp[(size_t)k * ma], ma * sizeof(double));
   float ms = 0;
   PS_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
@@ -556,7 +554,8 @@ int ps_close(ps_ctx* c) {
   free_problem(c);
   if (c->scratch) hipFree(c->scratch);
   if (c->counter) hipFree(c->counter);
-  for (void* q : {(void*)c->d_items_p, (void*)c->d_items_n, (void*)c->d_items_m, (void*)c->d_items_cw, (void*)c->d_out})
+  for (void* q : {(void*)c->d_items_p, (void*)c->d_items_n, (void*)c->d_items_m, (void*)c->d_items_cw, (void*)c->d_out,
+                  (void*)c->d_a0, (void*)c->d_afinal})
     if (q) hipFree(q);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -638,6 +637,137 @@ int ps_fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, con
   std::vector<double> cr(n_items);
   for (int i = 0; i < n_items; ++i) cr[i] = conw_r[i] * conw_r[i];
   return fit_items(c, n_items, prob, n_list, m_list, cr.data(), conw_r_dev, out);
+}
+
+int ps_fit_items_from(ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list,
+                      const double* conw_r, const double* conw_r_dev, const double* a0, int n_iter,
+                      ps_pair_result* out, double* a_final) {
+  if (!c || n_items < 0 || n_iter < 0 ||
+      (n_items && (!prob || !n_list || !m_list || !conw_r || !conw_r_dev || !out)))
+    return PS_E_ARG;
+  if (!c->prepared) return fail(c, PS_E_STATE, "ps_fit_items_from before ps_prepare");
+  std::vector<double> cr(n_items);
+  for (int i = 0; i < n_items; ++i) cr[i] = conw_r[i] * conw_r[i];
+  return fit_items(c, n_items, prob, n_list, m_list, cr.data(), conw_r_dev, out, a0, a_final, n_iter);
+}
+
+int ps_initial_params(const ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list,
+                      double* a0) {
+  if (!c || n_items < 0 || (n_items && (!prob || !n_list || !m_list || !a0))) return PS_E_ARG;
+  if (!c->prepared) return PS_E_STATE;
+  constexpr double kPi = 3.14159265358979323846;
+  const int C = c->ncoef, ma = c->ma, np = (int)c->hp.size();
+  for (int k = 0; k < n_items; ++k) {
+    const int p = prob[k], n = n_list[k], m = m_list[k];
+    if (p < 0 || p >= np || n < 1 || m < 1 || m > PS_N_POLES) return PS_E_ARG;
+    const HostProb& h = c->hp[p];
+    double* a = a0 + (size_t)k * ma;
+    // Start.cu:4346-4418, in the kernel's operation order
+    for (int i = 0; i < C; ++i) a[i] = h.cg_first[i];
+    const double period = 1.0 / (h.f_start - (double)(n - 1) * h.f_step);
+    const double d2r = kPi / 180.0;
+    a[C] = d2r * 90 - d2r * h.beta_pole[m - 1];
+    a[C + 1] = d2r * h.lambda_pole[m - 1];
+    a[C + 2] = (24.0 * 2.0 * kPi) / period;
+    a[C + 3] = h.par[0];
+    a[C + 4] = h.par[1];
+    a[C + 5] = h.par[2];
+    a[C + 6] = h.lcl;
+    a[C + 7] = 1;
+  }
+  return PS_OK;
+}
+
+int ps_free_params(const ps_ctx* c, int* ia) {
+  if (!c || !ia) return PS_E_ARG;
+  if (!c->prepared) return PS_E_STATE;
+  std::memcpy(ia, c->ia.data(), c->ia.size() * sizeof(int));
+  return PS_OK;
+}
+
+int ps_warm_n_features(void) { return PS_WARM_FEATURES; }
+
+int ps_warm_features(const ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list,
+                     float* out) {
+  if (!c || n_items < 0 || (n_items && (!prob || !n_list || !m_list || !out))) return PS_E_ARG;
+  if (!c->prepared) return PS_E_STATE;
+  const int np = (int)c->hp.size();
+  constexpr double kPi = 3.14159265358979323846;
+  // per problem: the lightcurves that enter the features (the first 8 curves
+  // spanning at most 2 days with at least 16 points) and the aspect summary
+  struct Sel {
+    std::vector<int> first, count;
+    double me[3] = {0, 0, 0}, ms[3] = {0, 0, 0}, mphase = 0;
+  };
+  std::vector<Sel> sel(np);
+  for (int p = 0; p < np; ++p) {
+    const HostProb& h = c->hp[p];
+    const int N = h.ndata;
+    int off = 0, used = 0;
+    for (size_t ci = 0; ci < h.l_points.size() && (int)sel[p].first.size() < PS_WARM_CURVES; ++ci) {
+      const int lp = h.l_points[ci];
+      if (lp >= 16 && h.tim[off + lp - 1] - h.tim[off] <= 2.0) {
+        sel[p].first.push_back(off);
+        sel[p].count.push_back(lp);
+        for (int j = off; j < off + lp; ++j) {
+          for (int d = 0; d < 3; ++d) {
+            sel[p].me[d] += h.ee[(size_t)d * N + j];
+            sel[p].ms[d] += h.ee0[(size_t)d * N + j];
+          }
+          const double cs = h.ee[j] * h.ee0[j] + h.ee[N + j] * h.ee0[N + j] + h.ee[2 * N + j] * h.ee0[2 * N + j];
+          sel[p].mphase += std::acos(std::fmax(-1.0, std::fmin(1.0, cs)));
+        }
+        used += lp;
+      }
+      off += lp;
+    }
+    if (used > 0) {
+      for (int d = 0; d < 3; ++d) {
+        sel[p].me[d] /= used;
+        sel[p].ms[d] /= used;
+      }
+      sel[p].mphase /= used;
+    }
+  }
+  for (int k = 0; k < n_items; ++k) {
+    const int p = prob[k];
+    if (p < 0 || p >= np || n_list[k] < 1 || m_list[k] < 1 || m_list[k] > PS_N_POLES) return PS_E_ARG;
+    const HostProb& h = c->hp[p];
+    const Sel& S = sel[p];
+    float* f = out + (size_t)k * PS_WARM_FEATURES;
+    for (int i = 0; i < PS_WARM_FEATURES; ++i) f[i] = 0.f;
+    const double period_h = 1.0 / (h.f_start - (double)(n_list[k] - 1) * h.f_step);
+    const double turns_per_day = 24.0 / period_h;
+    for (size_t ci = 0; ci < S.first.size(); ++ci) {
+      double sum[PS_WARM_BINS] = {0}, mean = 0;
+      int cnt[PS_WARM_BINS] = {0};
+      const int j0 = S.first[ci], lp = S.count[ci];
+      for (int j = j0; j < j0 + lp; ++j) mean += h.bright[j];
+      mean /= lp;
+      for (int j = j0; j < j0 + lp; ++j) {
+        double ph = h.tim[j] * turns_per_day;
+        ph -= std::floor(ph);
+        int b = (int)(ph * PS_WARM_BINS);
+        b = b < 0 ? 0 : (b >= PS_WARM_BINS ? PS_WARM_BINS - 1 : b);
+        sum[b] += h.bright[j] / mean - 1.0;
+        cnt[b] += 1;
+      }
+      for (int b = 0; b < PS_WARM_BINS; ++b)
+        f[ci * PS_WARM_BINS + b] = cnt[b] ? (float)(sum[b] / cnt[b]) : 0.f;
+    }
+    const int base = PS_WARM_CURVES * PS_WARM_BINS;
+    const double be = h.beta_pole[m_list[k] - 1] * kPi / 180.0, la = h.lambda_pole[m_list[k] - 1] * kPi / 180.0;
+    f[base + 0] = (float)(std::cos(be) * std::cos(la));
+    f[base + 1] = (float)(std::cos(be) * std::sin(la));
+    f[base + 2] = (float)std::sin(be);
+    for (int d = 0; d < 3; ++d) {
+      f[base + 3 + d] = (float)S.me[d];
+      f[base + 6 + d] = (float)S.ms[d];
+    }
+    f[base + 9] = (float)S.mphase;
+    f[base + 10] = (float)std::log(period_h / 10.0);
+  }
+  return PS_OK;
 }
 
 int ps_last_stats(const ps_ctx* c, double* kernel_ms, int* launches, long long* pairs, long long* vis_pairs,

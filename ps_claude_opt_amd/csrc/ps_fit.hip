@@ -901,10 +901,18 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
     L.cgg[C] = d2r * 90 - d2r * P.beta_pole[m - 1];
     L.cgg[C + 1] = d2r * P.lambda_pole[m - 1];
     L.cgg[C + 2] = (24.0 * 2.0 * kPi) / period;
-    L.cgg[C + 3] = P.par[0];
-    L.cgg[C + 4] = P.par[1];
-    L.cgg[C + 5] = P.par[2];
-    L.cgg[C + 6] = P.lcl;
+    if (A.item_a0) {
+      const double* a0 = A.item_a0 + (size_t)item * ma;
+      L.cgg[C + 3] = a0[C + 3];
+      L.cgg[C + 4] = a0[C + 4];
+      L.cgg[C + 5] = a0[C + 5];
+      L.cgg[C + 6] = a0[C + 6];
+    } else {
+      L.cgg[C + 3] = P.par[0];
+      L.cgg[C + 4] = P.par[1];
+      L.cgg[C + 5] = P.par[2];
+      L.cgg[C + 6] = P.lcl;
+    }
     L.cgg[C + 7] = 1;
     S.alamda = -1;
     S.niter = 0;
@@ -916,12 +924,15 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
     S.ochisq = 0;
     S.nvis = 0;
   }
-  for (int i = tid; i < C; i += WG) L.cgg[i] = P.cg_first[i];
+  for (int i = tid; i < C; i += WG) L.cgg[i] = A.item_a0 ? A.item_a0[(size_t)item * ma + i] : P.cg_first[i];
   __syncthreads();
   for (;;) {
     // Iter1Begin (Start.cu:4421-4462)
     const int niter = S.niter;
-    const bool is_niter = ((niter < P.n_iter_max) && (S.iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min);
+    const int n_iter_max = A.n_iter_fixed > 0 ? A.n_iter_fixed : P.n_iter_max;
+    const double iter_diff_max = A.n_iter_fixed > 0 ? 0.0 : P.iter_diff_max;
+    const int n_iter_min = A.n_iter_fixed > 0 ? 0 : P.n_iter_min;
+    const bool is_niter = ((niter < n_iter_max) && (S.iter_diff > iter_diff_max)) || (niter < n_iter_min);
     if (!is_niter) break;
     const bool is_alambda = S.alamda < 0;
     __syncthreads();
@@ -1038,6 +1049,8 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
     o[7] = 1.0;
     o[8] = (double)S.nvis;
   }
+  if (A.out_a)
+    for (int i = tid; i < ma; i += WG) A.out_a[(size_t)item * ma + i] = L.cgg[i];
   __syncthreads();
 }
 

@@ -55,6 +55,11 @@ struct PsFitArgs {
   double* scratch;      // per-workgroup scratch slabs (ps_slab layout, sized for the largest problem)
   long long scratch_stride;
   double* out;          // [n_items][PS_OUT_STRIDE]
+  const double* item_a0;  // [n_items][ma] initial shape and phase-function parameters, or null (the
+                          // reference's initialisation); pole, rotation rate and cls always come from (n, m)
+  double* out_a;        // [n_items][ma] final parameters, or null
+  int n_iter_fixed;     // > 0: exactly this many LM iterations (the reference's stopCondition > 1 mode)
+                        // instead of each problem's own stop condition
   double conw_r2_rows;  // ps_eval only: sigr2 of the convexity rows
   int batch;            // 1: items of several problems (read from probs), 0: every item is on P0
 };

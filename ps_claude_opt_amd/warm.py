@@ -1,0 +1,302 @@
+"""Learned warm start of the period scan.
+
+A small MLP (3 hidden layers of 256 GELU units, 52 outputs) predicts, for
+each (trial frequency, initial pole) pair, how far that pair's LM fit moves
+the shape coefficients (parameters 2..C) and phase-function / Lambert
+parameters (A, D, k, log cl) from the reference's initialisation (the
+ellipsoid of ellfit.c and the input file's phase function, Start.cu:
+4346-4418), from the lightcurves folded at the trial period, the pole and
+the viewing geometry (features: ps_warm_features in include/ps_mi355x.h).
+The fit then starts at initialisation + prediction; the LM iterations
+themselves stay FP64 and unchanged.  Only free parameters move.
+
+The network is trained on the GPU by the library's own kernels (BF16 operands,
+FP32 accumulation, Adam) on converged fits that the plain scan produces on
+seeded synthetic asteroids, and evaluated in FP32.  The plain cold-start scan
+stays the judge: ``warm_scan`` refits the best frequencies of a warm scan
+cold, so every reported winner is a cold fit.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import json
+import os
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+
+N_HIDDEN = 256
+N_LAYERS = 3
+
+
+class Mlp:
+    """The network kernels through the C ABI (ps_mlp_*)."""
+
+    def __init__(self, scan, n_in: int, n_out: int, n_hidden: int = N_HIDDEN, n_layers: int = N_LAYERS,
+                 seed: int = 0):
+        self._lib = scan._lib
+        self._scan = scan
+        h = C.c_void_p()
+        scan._check(self._lib.ps_mlp_create(scan._h, n_in, n_hidden, n_layers, n_out, seed, C.byref(h)))
+        self._h = h
+        self.n_in, self.n_out, self.n_hidden, self.n_layers = n_in, n_out, n_hidden, n_layers
+
+    def close(self):
+        if self._h:
+            self._lib.ps_mlp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_params(self) -> int:
+        return int(self._lib.ps_mlp_n_params(self._h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        self._scan._check(self._lib.ps_mlp_get_params(self._h, _abi.fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self._scan._check(self._lib.ps_mlp_set_params(self._h, _abi.fptr(p)))
+
+    def forward(self, x: np.ndarray, bf16: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        self._scan._check(self._lib.ps_mlp_forward(self._h, x.shape[0], _abi.fptr(x), _abi.fptr(y), int(bf16)))
+        return y
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float(0.0)
+        self._scan._check(self._lib.ps_mlp_loss_grad(self._h, x.shape[0], _abi.fptr(x), _abi.fptr(t), C.byref(loss),
+                                                     _abi.fptr(g)))
+        return float(loss.value), g
+
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int, batch: int = 512, lr: float = 1e-3,
+              seed: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        losses = np.zeros(max(epochs, 1), np.float32)
+        self._scan._check(self._lib.ps_mlp_train(self._h, x.shape[0], _abi.fptr(x), _abi.fptr(t), epochs, batch,
+                                                 lr, seed, _abi.fptr(losses)))
+        return losses[:epochs]
+
+
+def targets_from_params(a: np.ndarray, ncoef: int) -> np.ndarray:
+    """The predicted parameters of parameter vectors: 2..C, then A, D, k,
+    log cl (C-1 + 4 values; 52 at lmax = 6)."""
+    a = np.atleast_2d(a)
+    return np.concatenate([a[:, 1:ncoef], a[:, ncoef + 3:ncoef + 7]], axis=1)
+
+
+@dataclasses.dataclass
+class WarmStart:
+    """A trained warm-start network with its standardisation."""
+    params: np.ndarray
+    x_mean: np.ndarray
+    x_std: np.ndarray
+    y_mean: np.ndarray
+    y_std: np.ndarray
+    y_lo: Optional[np.ndarray] = None  # predictions are clipped to the training targets' range
+    y_hi: Optional[np.ndarray] = None
+    shrink: Optional[np.ndarray] = None  # per output: prediction = mean + shrink * (network - mean)
+    n_hidden: int = N_HIDDEN
+    n_layers: int = N_LAYERS
+    meta: Optional[dict] = None
+
+    @property
+    def n_in(self) -> int:
+        return int(self.x_mean.size)
+
+    @property
+    def n_out(self) -> int:
+        return int(self.y_mean.size)
+
+    def save(self, path: str):
+        np.savez(path, params=self.params, x_mean=self.x_mean, x_std=self.x_std, y_mean=self.y_mean,
+                 y_std=self.y_std, y_lo=self.y_lo, y_hi=self.y_hi, shrink=self.shrink,
+                 shape=np.array([self.n_hidden, self.n_layers]),
+                 meta=np.array(json.dumps(self.meta or {})))
+
+    @classmethod
+    def load(cls, path: str) -> "WarmStart":
+        d = np.load(path, allow_pickle=False)
+        return cls(params=d["params"], x_mean=d["x_mean"], x_std=d["x_std"], y_mean=d["y_mean"], y_std=d["y_std"],
+                   y_lo=d["y_lo"], y_hi=d["y_hi"], shrink=d["shrink"], n_hidden=int(d["shape"][0]),
+                   n_layers=int(d["shape"][1]),
+                   meta=json.loads(str(d["meta"])))
+
+    def network(self, scan) -> Mlp:
+        net = Mlp(scan, self.n_in, self.n_out, self.n_hidden, self.n_layers)
+        net.set_params(self.params)
+        return net
+
+    def predict(self, net: Mlp, feats: np.ndarray) -> np.ndarray:
+        """Predicted parameter moves (raw units) of feature rows, FP32."""
+        y = net.forward((feats - self.x_mean) / self.x_std) * self.y_std + self.y_mean
+        if self.shrink is not None:
+            y = self.y_mean + self.shrink * (y - self.y_mean)
+        if self.y_lo is not None:
+            y = np.clip(y, self.y_lo, self.y_hi)
+        return y
+
+    def initial_params(self, scan, net: Mlp, prob, n_list, m_list) -> np.ndarray:
+        """Warm initial parameter vectors [items][ma]: the reference's
+        initialisation with the free shape and phase-function parameters
+        moved by the network's FP32 prediction."""
+        feats = scan.warm_features(prob, n_list, m_list)
+        y = self.predict(net, feats)
+        a0 = scan.initial_params(prob, n_list, m_list)
+        Cn = scan.info["ncoef"]
+        ia = scan.free_params()
+        for j in range(Cn - 1):
+            if ia[1 + j]:
+                a0[:, 1 + j] += y[:, j]
+        for j in range(4):
+            if ia[Cn + 3 + j]:
+                a0[:, Cn + 3 + j] += y[:, Cn - 1 + j]
+        return a0
+
+
+def fit_standardisation(x: np.ndarray, y: np.ndarray):
+    xm, xs = x.mean(0), x.std(0)
+    ym, ys = y.mean(0), y.std(0)
+    xs = np.where(xs > 1e-6, xs, 1.0).astype(np.float32)
+    ys = np.where(ys > 1e-9, ys, 1.0).astype(np.float32)
+    return xm.astype(np.float32), xs, ym.astype(np.float32), ys
+
+
+def make_training_set(scan, n_pairs_per_problem: int, conw_r: Sequence[float], seed: int = 0):
+    """Cold fits of seeded random (frequency, pole) pairs of every problem of
+    the scan's batch; returns (features, targets = final - initial
+    parameters, results, problem index) of the fits that converged to finite
+    parameters."""
+    rng = np.random.default_rng(seed)
+    prob, nl, ml = [], [], []
+    for i, b in enumerate(scan.batch_info):
+        n = rng.integers(1, b["n_max"] + 1, n_pairs_per_problem)
+        prob += [i] * n_pairs_per_problem
+        nl += list(n)
+        ml += list(rng.integers(1, 11, n_pairs_per_problem))
+    prob, nl, ml = np.array(prob), np.array(nl), np.array(ml)
+    cw = np.asarray(conw_r, np.float64)[prob]
+    res, a_final = scan.fit_items_from(prob, nl, ml, cw, cw, None, want_final=True)
+    feats = scan.warm_features(prob, nl, ml)
+    a_init = scan.initial_params(prob, nl, ml)
+    Cn = scan.info["ncoef"]
+    tg = (targets_from_params(a_final, Cn) - targets_from_params(a_init, Cn)).astype(np.float32)
+    ok = np.isfinite(tg).all(1) & np.isfinite(res["dev"]) & (res["dev"] < 1e30)
+    # fits that ran away (phase-function parameters in the 1e20s and beyond
+    # with a finite rms) are no examples of where a fit goes: keep rows whose
+    # every move lies within 10 interquartile ranges of the quartiles
+    q1, q3 = np.percentile(tg[ok], 25, axis=0), np.percentile(tg[ok], 75, axis=0)
+    lo, hi = q1 - 10 * (q3 - q1) - 1e-6, q3 + 10 * (q3 - q1) + 1e-6
+    ok &= ((tg >= lo) & (tg <= hi)).all(1)
+    return feats[ok], tg[ok], res[ok], prob[ok]
+
+
+def train_warm_start(scan, feats: np.ndarray, targets: np.ndarray, groups: Optional[np.ndarray] = None,
+                     epochs: int = 60, batch: int = 512, lr: float = 1e-3, seed: int = 0, val_frac: float = 0.2,
+                     patience: int = 8, meta: Optional[dict] = None):
+    """Trains the network on the GPU (BF16, Adam).  The rows of a held-out
+    fraction of the groups (asteroids) pick the epoch (early stopping on the
+    FP32 validation loss) and a per-output shrinkage factor towards the mean
+    move, fitted in closed form on the same rows.  Returns (WarmStart,
+    per-epoch training losses)."""
+    rng = np.random.default_rng(seed)
+    if groups is None:
+        groups = np.arange(len(feats))
+    ug = np.unique(groups)
+    val_g = rng.choice(ug, max(1, int(round(val_frac * len(ug)))), replace=False)
+    val = np.isin(groups, val_g)
+    tr = ~val
+    xm, xs, ym, ys = fit_standardisation(feats[tr], targets[tr])
+    xt, yt = (feats[tr] - xm) / xs, (targets[tr] - ym) / ys
+    xv, yv = (feats[val] - xm) / xs, (targets[val] - ym) / ys
+    net = Mlp(scan, feats.shape[1], targets.shape[1], seed=seed)
+    best, best_loss, since = net.get_params(), np.inf, 0
+    losses = []
+    for e in range(epochs):
+        losses.append(float(net.train(xt, yt, 1, batch, lr, seed + 1 + e)[0]))
+        vl = float(np.mean((net.forward(xv) - yv) ** 2))
+        if vl < best_loss:
+            best, best_loss, since = net.get_params(), vl, 0
+        else:
+            since += 1
+            if since >= patience:
+                break
+    net.set_params(best)
+    pv = net.forward(xv)  # standardised predictions on the validation rows
+    num = (pv * yv).sum(0)
+    den = (pv * pv).sum(0)
+    shrink = np.clip(np.where(den > 0, num / np.maximum(den, 1e-30), 0.0), 0.0, 1.0).astype(np.float32)
+    lo, hi = np.percentile(targets[tr], 0.5, axis=0), np.percentile(targets[tr], 99.5, axis=0)
+    ws = WarmStart(best, xm, xs, ym, ys, lo.astype(np.float32), hi.astype(np.float32), shrink,
+                   meta=dict(meta or {}, epochs_run=len(losses), batch=batch, lr=lr, val_loss=best_loss,
+                             train_rows=int(tr.sum()), val_rows=int(val.sum())))
+    net.close()
+    return ws, np.array(losses)
+
+
+def best_lines(res: np.ndarray, n_freq: int):
+    """Index of the best pole per frequency (start_CUDA.cu:1200-1210) for
+    results ordered frequency-major, 10 poles each."""
+    dev = res["dev"].reshape(n_freq, 10)
+    best = np.zeros(n_freq, np.int64)
+    for f in range(n_freq):
+        b = 0
+        for p in range(1, 10):
+            if not np.isnan(dev[f, p]) and not (dev[f, p] >= dev[f, b]):
+                b = p
+        best[f] = b
+    return best
+
+
+def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, n_start: int, n_count: int,
+              n_iter: int, refit: int = 10):
+    """Warm-started scan of problem 0: every pair runs n_iter LM iterations
+    from the network's prediction (ws None: from the reference's start);
+    then the `refit` frequencies with the lowest rms are fitted again cold,
+    with the problem's own stop condition (all 10 poles), and those cold
+    results replace the truncated ones.  Returns (results [n_count*10]
+    frequency-major, refitted frequency indices)."""
+    n = np.repeat(np.arange(n_start, n_start + n_count), 10)
+    m = np.tile(np.arange(1, 11), n_count)
+    p = np.zeros(n.size, np.int32)
+    a0 = ws.initial_params(scan, net, p, n, m) if ws is not None else None
+    res, _ = scan.fit_items_from(p, n, m, conw_r, conw_r, a0, n_iter=n_iter)
+    best = best_lines(res, n_count)
+    rms = res["dev"].reshape(n_count, 10)[np.arange(n_count), best]
+    order = np.argsort(np.where(np.isnan(rms), np.inf, rms), kind="stable")[:refit]
+    fr = np.sort(order)
+    if fr.size:
+        n2 = np.repeat(n_start + fr, 10)
+        m2 = np.tile(np.arange(1, 11), fr.size)
+        cold, _ = scan.fit_items_from(np.zeros(n2.size, np.int32), n2, m2, conw_r, conw_r, None)
+        for k, f in enumerate(fr):
+            res[f * 10:(f + 1) * 10] = cold[k * 10:(k + 1) * 10]
+    return res, n_start + fr
+
+
+def best_line(res: np.ndarray, n_start: int, n_count: int) -> dict:
+    """The overall best (lowest rms) line of a scan's results."""
+    best = best_lines(res, n_count)
+    rows = np.arange(n_count) * 10 + best
+    rms = res["dev"][rows]
+    f = int(np.nanargmin(np.where(np.isnan(rms), np.inf, rms)))
+    r = res[rows[f]]
+    return dict(freq_index=n_start + f, pole=int(best[f]) + 1, period_h=24.0 * float(r["period"]),
+                rms=float(r["dev"]), lambda_deg=float(r["lambda_deg"]), beta_deg=float(r["beta_deg"]),
+                chisq=float(r["chisq"]))

@@ -1,0 +1,126 @@
+"""Trains the lmax = 6 warm-start network on the GPU and reports what it
+gains on held-out asteroids.
+
+    python -m ps_claude_opt_amd.warm_train --asteroids 800 --pairs 40 \
+        --out ps_claude_opt_amd/data/warm_lmax6.npz
+
+Training data: cold LM fits (the input file's own 50-iteration stop mode) of
+seeded random (frequency, pole) pairs of config-2-style synthetic asteroids
+(seeds 20261203 + 4000 + i: disjoint from the benchmark's asteroid), each
+fitted with its initial convexity weight conw / escl^2.  Features and targets
+as in warm.py; the network is trained by the library's kernels (BF16 / FP32
+accumulation, Adam).  Validation: on held-out asteroids, the rms reached by
+a few LM iterations from the network's start versus from the reference's
+start, both against the full cold fit.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from multiprocessing import Pool
+
+import numpy as np
+
+
+def _write_one(args):
+    path, index = args
+    from ps_claude_opt_amd import synth
+
+    synth.write_asteroid(path, "c2", synth.config_seed(4, index))
+    return path
+
+
+def _one_thread():
+    # each worker renders with one BLAS thread (the workers are the parallelism)
+    try:
+        from threadpoolctl import threadpool_limits
+
+        threadpool_limits(1)
+    except ImportError:
+        pass
+
+
+def generate(directory: str, first: int, count: int, workers: int):
+    os.makedirs(directory, exist_ok=True)
+    jobs = [(os.path.join(directory, f"a{first + i:05d}.in"), first + i) for i in range(count)]
+    with Pool(workers, initializer=_one_thread) as pool:
+        return pool.map(_write_one, jobs)
+
+
+def initial_conw(scan) -> np.ndarray:
+    return np.array([b["conw"] / b["escl"] / b["escl"] for b in scan.batch_info])
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--asteroids", type=int, default=800)
+    ap.add_argument("--validation", type=int, default=16)
+    ap.add_argument("--pairs", type=int, default=40, help="training pairs per asteroid")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--work-dir", default="gpurun_out/warm")
+    ap.add_argument("--out", default="gpurun_out/warm/warm_lmax6.npz")
+    args = ap.parse_args(argv)
+
+    import ps_claude_opt_amd as ps
+    from ps_claude_opt_amd import warm
+
+    t0 = time.time()
+    train_paths = generate(os.path.join(args.work_dir, "train"), 0, args.asteroids, args.workers)
+    val_paths = generate(os.path.join(args.work_dir, "val"), 100000, args.validation, args.workers)
+    print(f"# generated {len(train_paths)} + {len(val_paths)} asteroids in {time.time() - t0:.1f} s", flush=True)
+    report = dict(asteroids=args.asteroids, pairs_per_asteroid=args.pairs, epochs=args.epochs, batch=args.batch,
+                  lr=args.lr)
+    with ps.Scan(0) as s:
+        t0 = time.time()
+        s.prepare_files(train_paths)
+        feats, tg, res, prob = warm.make_training_set(s, args.pairs, initial_conw(s), seed=11)
+        report.update(training_pairs=int(len(feats)), fit_s=round(time.time() - t0, 2))
+        print(f"# {len(feats)} training pairs in {time.time() - t0:.1f} s", flush=True)
+        t0 = time.time()
+        ws, losses = warm.train_warm_start(s, feats, tg, prob, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                                           meta=dict(features=int(feats.shape[1]), outputs=int(tg.shape[1]),
+                                                     asteroids=args.asteroids, seed_base="config_seed(4, i)"))
+        report.update(train_s=round(time.time() - t0, 2), loss_first=float(losses[0]), loss_last=float(losses[-1]),
+                      epochs_run=len(losses), val_loss=ws.meta["val_loss"],
+                      shrink_mean=float(ws.shrink.mean()))
+        print(f"# trained in {time.time() - t0:.1f} s: loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+        os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+        ws.save(args.out)
+        # validation on held-out asteroids
+        s.prepare_files(val_paths)
+        rng = np.random.default_rng(99)
+        k = 200
+        pr = np.repeat(np.arange(len(val_paths)), k)
+        nl = np.concatenate([rng.integers(1, b["n_max"] + 1, k) for b in s.batch_info])
+        ml = rng.integers(1, 11, pr.size)
+        cw = initial_conw(s)[pr]
+        full, _ = s.fit_items_from(pr, nl, ml, cw, cw, None)
+        net = ws.network(s)
+        a0 = ws.initial_params(s, net, pr, nl, ml)
+        net.close()
+        val = {}
+        for it in (5, 10, 15, 20, 30):
+            c, _ = s.fit_items_from(pr, nl, ml, cw, cw, None, n_iter=it)
+            w, _ = s.fit_items_from(pr, nl, ml, cw, cw, a0, n_iter=it)
+            ok = np.isfinite(full["dev"]) & (full["dev"] < 1e30)
+            gc = np.abs(c["dev"] - full["dev"])[ok]
+            gw = np.abs(w["dev"] - full["dev"])[ok]
+            val[str(it)] = dict(cold_median_gap=float(np.median(gc)), warm_median_gap=float(np.median(gw)),
+                                cold_within_5e4=float(np.mean(gc <= 5e-4)), warm_within_5e4=float(np.mean(gw <= 5e-4)))
+            print(f"# {it} iterations: |dev - dev_full| median cold {np.median(gc):.2e} warm {np.median(gw):.2e}; "
+                  f"within 5e-4: cold {np.mean(gc <= 5e-4):.2f} warm {np.mean(gw <= 5e-4):.2f}", flush=True)
+        report["validation"] = val
+    with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:
+        json.dump(report, f, indent=1)
+    print(json.dumps(report), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
