@@ -35,7 +35,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # HBM bytes per pair-fit of the fit kernel, from FETCH_SIZE / WRITE_SIZE
 # counter passes scaled by a plain copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ps_fit_kernel_traffic.json")
-WARM_WEIGHTS = os.path.join(ROOT, "ps_claude_opt_amd", "data", "warm_lmax6.npz")
+WARM_WEIGHTS = os.path.join(ROOT, "ps_claude_opt_amd", "data", "warm_lmax6.npz")  # warm.DEFAULT_WEIGHTS
 
 
 def measured_traffic(preset: str, pairs_per_launch: float):
@@ -382,15 +382,30 @@ def cpu_baseline(path: str, conw_r: float, n_max: int, n_pairs: int) -> dict:
     ml = rng.integers(1, 11, n_pairs)
     cores = _cores()
     _, wall = O.fit_pairs(nl, ml, conw_r, conw_r, threads=cores)
+    n1 = max(1, n_pairs // 32)
+    _, wall1 = O.fit_pairs(nl[:n1], ml[:n1], conw_r, conw_r, threads=1)
     return {
         "value": round(n_pairs / wall, 4),
         "unit": "pair-fits/s",
         "cores": cores,
         "kind": "port",
         "sample": f"{n_pairs} seeded random (frequency, pole) pairs of the same scan, {wall:.1f} s wall "
-                  f"on {cores} threads of the host CPU; the full scan would take "
+                  f"on {cores} threads of the host CPU ({_cpu_model()}); the full scan would take "
                   f"{n_max * 10 / (n_pairs / wall) / 3600:.2f} h",
+        "single_core_value": round(n1 / wall1, 4),
+        "single_core_sample": f"the first {n1} of those pairs on one thread, {wall1:.1f} s",
     }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 if __name__ == "__main__":

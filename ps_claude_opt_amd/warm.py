@@ -30,6 +30,8 @@ from . import _abi
 
 N_HIDDEN = 256
 N_LAYERS = 3
+# Weights trained on seeded lmax-6 asteroids (288 facets, 54 free parameters).
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "warm_lmax6.npz")
 
 
 class Mlp:
