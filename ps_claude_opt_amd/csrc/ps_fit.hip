@@ -5,28 +5,32 @@
 //
 // Per mrqcof evaluation (reference Start.cu:2262-2828):
 //   curv          facet areas Area_f = Darea_f exp(sum P_l^m (c cos + s sin))
-//   phase 1       per 16-point tile and 4-facet step, lane l owns point
-//                 (l & 15) and facet (4k + (l >> 4)); it evaluates the
-//                 scattering law and the derivative sums on the VALU and
-//                 feeds the facet weight w = Area * S as the A operand of
-//                 v_mfma_f64_16x16x4f64 against the shared basis table
-//                 (B operand), which yields the shape Jacobian of 16 points
-//                 x 16 coefficients per block.  Raw Jacobian rows go to a
-//                 per-workgroup scratch (L2-resident).
-//   curve sums    column sums d and brightness sum L of the curve
+//   phase 1       over all 16-point tiles of all curves, round-robin over
+//                 the waves; per 4-facet step lane l owns point (l & 15) and
+//                 facet (4k + (l >> 4)); it evaluates the scattering law and
+//                 the derivative sums on the VALU and feeds the facet weight
+//                 w = Area * S as the A operand of v_mfma_f64_16x16x4f64
+//                 against the basis table (B operand, read through L2),
+//                 which yields the shape Jacobian of 16 points x 16
+//                 coefficients per block.  Raw Jacobian rows and per-tile
+//                 column sums go to the workgroup's scratch slab.
+//   curve sums    column sums d and brightness sum L of each curve, from
+//                 its tile sums
 //   phase 2       relative renormalisation J' = c (J - y d / L), residual
-//                 r; the augmented product [J'|r]^T W [J'|r] is one FP64
-//                 MFMA SYRK whose lower blocks stay in registers: it gives
+//                 r; the augmented product [J'|r]^T W [J'|r] is an FP64
+//                 MFMA SYRK split into tile chains, each wave owning lower
+//                 16x16 block pairs of one chain in registers: it gives
 //                 alpha, beta and chi^2 at once.
-// mrqmin: full-pivot Gauss-Jordan in LDS, accept / reject, Iter2, FinishPole
-// (Start.cu:2838-3021, 300-392, 5129-5248, 5056-5125).
+// mrqmin: full-pivot Gauss-Jordan in LDS by all waves, accept / reject,
+// Iter2, FinishPole (Start.cu:2838-3021, 300-392, 5129-5248, 5056-5125).
 //
 // Operation order is fixed (see the CPU oracle): facet partial sums by
 // facet mod 4 combined (P0+P1)+(P2+P3); the MFMA gather is a fused
-// multiply-add chain over facets in ascending order; every element of the
-// augmented normal matrix is one fused multiply-add chain over all points in
-// order (the 16x16 block pairs are split across the waves, every wave walks
-// every tile).
+// multiply-add chain over facets in ascending order; tile column sums are
+// four partials over points p mod 4 combined (G0+G1)+(G2+G3), curve sums add
+// the tile sums in order; each element of the augmented normal matrix is
+// the combination (C0+C1)+(C2+C3) (or C0+C1 above 64 columns) of fused
+// multiply-add chains over the tiles T = g (mod chains), points in order.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -359,27 +363,17 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int
   for (int b = 0; b < NBS; ++b) gat[b] = v4d{0.0, 0.0, 0.0, 0.0};
   // lane facet f = 4k + sl: normal/area at nrm[4f..4f+3], basis values of
   // shape columns 16b + ps at ytab[(16f + ps) * NBS + b].  Software
-  // pipeline: the normals are loaded one step ahead, the basis values (LDS
-  // or L2) two steps ahead.
+  // pipeline over two register sets (A: even steps, B: odd steps; the
+  // triangulation's step count F / 4 = 2 nrows^2 is even): the normals of a
+  // step are read from
+  // LDS one step ahead, its basis values from L2 as soon as the step two
+  // before has issued its MFMAs.  Two sets keep the pipeline free of
+  // register-rotation moves.
   const int NK = F / 4;
   const double* __restrict__ nr = L.nrm + 4 * sl;
   const double* __restrict__ yb = L.ytab + (size_t)(16 * sl + ps) * NBS;
-  double n0 = nr[0], n1 = nr[1], n2 = nr[2], ar = nr[3];
-  double ya[NBS], yn[NBS];
-#pragma unroll
-  for (int b = 0; b < NBS; ++b) {
-    ya[b] = yb[b];
-    yn[b] = yb[(NK > 1 ? (size_t)64 * NBS : 0) + b];
-  }
   unsigned nvis = 0;
-  for (int k = 0; k < NK; ++k) {
-    const int k1 = k + 1 < NK ? k + 1 : k;
-    const int k2 = k + 2 < NK ? k + 2 : k;
-    const double* nrn = nr + 16 * k1;
-    const double m0 = nrn[0], m1 = nrn[1], m2 = nrn[2], mar = nrn[3];
-    double yc[NBS];
-#pragma unroll
-    for (int b = 0; b < NBS; ++b) yc[b] = yb[(size_t)k2 * 64 * NBS + b];
+  auto step = [&](double n0, double n1, double n2, double ar, const double* y) {
     const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
     const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
     const bool vis = (mu > kTiny) && (mu0 > kTiny);
@@ -407,17 +401,33 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int
         t5 = fma(ar * mm, rd, t5);
       }
 #pragma unroll
-      for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, ya[b], gat[b]);
+      for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, y[b], gat[b]);
     }
-    n0 = m0;
-    n1 = m1;
-    n2 = m2;
-    ar = mar;
+  };
+  double a0 = nr[0], a1 = nr[1], a2 = nr[2], aar = nr[3];
+  double yA[NBS], yB[NBS];
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) {
-      ya[b] = yn[b];
-      yn[b] = yc[b];
-    }
+  for (int b = 0; b < NBS; ++b) {
+    yA[b] = yb[b];
+    yB[b] = yb[(NK > 1 ? (size_t)64 * NBS : 0) + b];
+  }
+  for (int k = 0; k < NK; k += 2) {
+    const int k1 = k + 1 < NK ? k + 1 : k;
+    const double* nb = nr + 16 * k1;
+    const double b0 = nb[0], b1 = nb[1], b2 = nb[2], bar = nb[3];
+    step(a0, a1, a2, aar, yA);
+    const int k2 = k + 2 < NK ? k + 2 : k;
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) yA[b] = yb[(size_t)k2 * 64 * NBS + b];
+    const double* na = nr + 16 * k2;
+    a0 = na[0];
+    a1 = na[1];
+    a2 = na[2];
+    aar = na[3];
+    if (k + 1 < NK) step(b0, b1, b2, bar, yB);  // odd step count: F = 4 (mod 8)
+    const int k3 = k + 3 < NK ? k + 3 : k1;
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) yB[b] = yb[(size_t)k3 * 64 * NBS + b];
   }
   br = xsum16_32(br);
   t4 = xsum16_32(t4);
@@ -1061,8 +1071,23 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
 // until it is empty (BATCH: items of several problems, described in global
 // memory; otherwise every item is on problem P0, passed as an argument).
 // ---------------------------------------------------------------------------
+// Register budget: the resident wavefronts per SIMD the allocation targets.
+// Up to 64 Jacobian columns a 4-wave fit fits in 168 VGPRs (three
+// workgroups per CU; the few spills are in per-iteration bookkeeping, not in
+// the facet, SYRK or Gauss-Jordan loops); wider fits and 8-wave workgroups
+// take up to 256 (two waves per SIMD).  PS_FIT_WAVES_PER_EU overrides it.
+template <int NB>
+constexpr int fit_waves_per_eu() {
+#ifdef PS_FIT_WAVES_PER_EU
+  return PS_FIT_WAVES_PER_EU;
+#else
+  return NW == 4 && NB <= 4 ? 3 : 2;
+#endif
+}
+
 template <int NB, int NBS, bool BATCH>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void ps_fit_kernel(PsFitArgs A) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(fit_waves_per_eu<NB>()))) void ps_fit_kernel(
+    PsFitArgs A) {
   extern __shared__ double lds_dyn[];
   __shared__ PairState S;
   __shared__ int item_s;

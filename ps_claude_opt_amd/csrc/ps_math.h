@@ -16,20 +16,31 @@
 
 namespace psm {
 
+// A polynomial or reduction constant.  In device code it is materialised in
+// scalar registers at its use: left to the optimiser, the constants of the
+// per-point sincos / exp2 are hoisted out of the kernel's loops into vector
+// registers that then stay live (or spilled) for the whole fit.
+PSM_HD double kc(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+s"(v));
+#endif
+  return v;
+}
+
 PSM_HD double exp_poly(double r) {
   // sum_{n=0}^{13} r^n / n!
-  double p = 0x1.6124613a86d09p-33;
-  p = fma(p, r, 0x1.1eed8eff8d898p-29);
-  p = fma(p, r, 0x1.ae64567f544e4p-26);
-  p = fma(p, r, 0x1.27e4fb7789f5cp-22);
-  p = fma(p, r, 0x1.71de3a556c734p-19);
-  p = fma(p, r, 0x1.a01a01a01a01ap-16);
-  p = fma(p, r, 0x1.a01a01a01a01ap-13);
-  p = fma(p, r, 0x1.6c16c16c16c17p-10);
-  p = fma(p, r, 0x1.1111111111111p-7);
-  p = fma(p, r, 0x1.5555555555555p-5);
-  p = fma(p, r, 0x1.5555555555555p-3);
-  p = fma(p, r, 0x1.0000000000000p-1);
+  double p = kc(0x1.6124613a86d09p-33);
+  p = fma(p, r, kc(0x1.1eed8eff8d898p-29));
+  p = fma(p, r, kc(0x1.ae64567f544e4p-26));
+  p = fma(p, r, kc(0x1.27e4fb7789f5cp-22));
+  p = fma(p, r, kc(0x1.71de3a556c734p-19));
+  p = fma(p, r, kc(0x1.a01a01a01a01ap-16));
+  p = fma(p, r, kc(0x1.a01a01a01a01ap-13));
+  p = fma(p, r, kc(0x1.6c16c16c16c17p-10));
+  p = fma(p, r, kc(0x1.1111111111111p-7));
+  p = fma(p, r, kc(0x1.5555555555555p-5));
+  p = fma(p, r, kc(0x1.5555555555555p-3));
+  p = fma(p, r, kc(0x1.0000000000000p-1));
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return p;
@@ -61,29 +72,29 @@ PSM_HD void sincos(double x, double* s, double* c) {
     return;
   }
   const double k = rint(x * 0x1.45f306dc9c883p-1);
-  double r = fma(-k, 0x1.921fb54400000p+0, x);
-  r = fma(-k, 0x1.0b4611a600000p-34, r);
-  r = fma(-k, 0x1.3198a2e000000p-69, r);
+  double r = fma(-k, kc(0x1.921fb54400000p+0), x);
+  r = fma(-k, kc(0x1.0b4611a600000p-34), r);
+  r = fma(-k, kc(0x1.3198a2e000000p-69), r);
   const double z = r * r;
-  double ps = -0x1.2f49b46814157p-57;
-  ps = fma(ps, z, 0x1.952c77030ad4ap-49);
-  ps = fma(ps, z, -0x1.ae7f3e733b81fp-41);
-  ps = fma(ps, z, 0x1.6124613a86d09p-33);
-  ps = fma(ps, z, -0x1.ae64567f544e4p-26);
-  ps = fma(ps, z, 0x1.71de3a556c734p-19);
-  ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
-  ps = fma(ps, z, 0x1.1111111111111p-7);
-  ps = fma(ps, z, -0x1.5555555555555p-3);
+  double ps = kc(-0x1.2f49b46814157p-57);
+  ps = fma(ps, z, kc(0x1.952c77030ad4ap-49));
+  ps = fma(ps, z, kc(-0x1.ae7f3e733b81fp-41));
+  ps = fma(ps, z, kc(0x1.6124613a86d09p-33));
+  ps = fma(ps, z, kc(-0x1.ae64567f544e4p-26));
+  ps = fma(ps, z, kc(0x1.71de3a556c734p-19));
+  ps = fma(ps, z, kc(-0x1.a01a01a01a01ap-13));
+  ps = fma(ps, z, kc(0x1.1111111111111p-7));
+  ps = fma(ps, z, kc(-0x1.5555555555555p-3));
   const double sn = fma(r * z, ps, r);
-  double pc = -0x1.6827863b97d97p-53;
-  pc = fma(pc, z, 0x1.ae7f3e733b81fp-45);
-  pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
-  pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
-  pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
-  pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
-  pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
-  pc = fma(pc, z, 0x1.5555555555555p-5);
-  pc = fma(pc, z, -0x1.0000000000000p-1);
+  double pc = kc(-0x1.6827863b97d97p-53);
+  pc = fma(pc, z, kc(0x1.ae7f3e733b81fp-45));
+  pc = fma(pc, z, kc(-0x1.93974a8c07c9dp-37));
+  pc = fma(pc, z, kc(0x1.1eed8eff8d898p-29));
+  pc = fma(pc, z, kc(-0x1.27e4fb7789f5cp-22));
+  pc = fma(pc, z, kc(0x1.a01a01a01a01ap-16));
+  pc = fma(pc, z, kc(-0x1.6c16c16c16c17p-10));
+  pc = fma(pc, z, kc(0x1.5555555555555p-5));
+  pc = fma(pc, z, kc(-0x1.0000000000000p-1));
   const double cs = fma(z, pc, 1.0);
   const long long q = (long long)k & 3;
   if (q == 0) { *s = sn; *c = cs; }
