@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "ps_math.h"
 #include "ps_kernels.h"
@@ -164,7 +165,7 @@ __device__ __forceinline__ Inv load_inv(const double* d) {
 }
 
 // Pair invariants of one parameter vector a[] (a[i] = parameter i+1).
-__device__ Inv make_inv(const PsDevProblem& P, const double* a) {
+__device__ __forceinline__ Inv make_inv(const PsDevProblem& P, const double* a) {
   const int C = P.ncoef;
   Inv v;
   v.omega = a[C + 2];
@@ -238,7 +239,7 @@ struct Lds {
 
 // Pointers into one workgroup's scratch slab.
 struct Slab {
-  double *jbuf, *ybuf, *tsum, *dsum, *part, *best;
+  double *jbuf, *ybuf, *tsum, *dsum, *part, *best, *pgeo;
 };
 
 __device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
@@ -250,10 +251,11 @@ __device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
   s.dsum = base + o.dsum;
   s.part = base + o.part;
   s.best = base + o.best;
+  s.pgeo = base + o.pgeo;
   return s;
 }
 
-__device__ Lds carve_lds(double* base, const PsDevProblem& P) {
+__device__ __forceinline__ Lds carve_lds(double* base, const PsDevProblem& P) {
   Lds L;
   const int K = P.mfit + 1;
   double* p = base;
@@ -275,7 +277,7 @@ __device__ Lds carve_lds(double* base, const PsDevProblem& P) {
 }
 
 // Copies the facet normals into the workgroup's LDS once per launch.
-__device__ void stage_tables(const PsDevProblem& P, const Lds& L) {
+__device__ __forceinline__ void stage_tables(const PsDevProblem& P, const Lds& L) {
   const int F = P.nfac;
   for (int f = threadIdx.x; f < F; f += WG) {
     L.nrm[4 * f + 0] = P.nx[f];
@@ -292,7 +294,7 @@ __device__ __forceinline__ double ybasis(const PsDevProblem& P, const double* yt
 }
 
 // Facet areas of parameter vector a (Start.cu:421-528) into nrm[4f+3].
-__device__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) {
+__device__ __forceinline__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) {
   const int F = P.nfac, Lm = P.lmax, M = P.mmax;
   for (int f = tid_opaque(); f < F; f += WG) {
     double g = 0.0;
@@ -334,27 +336,27 @@ __device__ __forceinline__ PointScalars point_scalars(const PsDevProblem& P, con
 
 // Phase 1 for one 16-point tile of one curve, executed by one wave.
 template <int NB, int NBS>
-__device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int lp, int t, double* jbuf,
-                                double* ybuf, double* tsum, double* rowbuf) {
+__device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, const double* pgeo, int p0, int lp,
+                                                int t, double* jbuf, double* ybuf, double* tsum, double* rowbuf) {
   const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
   const int F = P.nfac, QS = P.qs, C = P.ncoef;
   const int pin = t * 16 + ps;
   const bool valid = pin < lp;
   const int pt = p0 + (valid ? pin : 0);
   (void)NB;
-  // point geometry: only the body-frame directions stay live in the facet
-  // loop; the rest is recomputed (identically) for the row afterwards
-  double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0}, cl, cls;
-  {
-    const Inv v = load_inv(L.inv);
-    cl = v.cl;
-    cls = v.cls;
-    if (valid) {
-      double scr[3];
-      const PointScalars q = point_scalars(P, v, pt);
-      body_vectors(v, q.cf, q.sf, q.tt, P.ex[pt], P.ey[pt], P.ez[pt], g, scr, scr, scr);
-      body_vectors(v, q.cf, q.sf, q.tt, P.sx[pt], P.sy[pt], P.sz[pt], g0, scr, scr, scr);
-    }
+  // point geometry (the evaluation's per-point table): only the body-frame
+  // directions stay live in the facet loop; padding lanes see nothing
+  const int N = P.ndata;
+  double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0};
+  const double cl = L.inv[5], cls = L.inv[6];
+  if (valid) {
+    const double* o = pgeo + pt;
+    g[0] = o[5 * N];
+    g[1] = o[6 * N];
+    g[2] = o[7 * N];
+    g0[0] = o[8 * N];
+    g0[1] = o[9 * N];
+    g0[2] = o[10 * N];
   }
   double br = 0.0, t4 = 0.0, t5 = 0.0;
   double U0 = 0.0, U1 = 0.0, U2 = 0.0, V0 = 0.0, V1 = 0.0, V2 = 0.0;
@@ -445,9 +447,17 @@ __device__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, int p0, int
     const int z = lane >> 6;
     const Inv v = load_inv(L.inv + z);
     const int pq = pt + z;
-    PointScalars q = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (valid) q = point_scalars(P, v, pq);
-    const double sf = q.sf, cf = q.cf, tt = q.tt, scale = q.scale, ff = q.ff, d2 = q.d2, alph = q.alph;
+    double sf = 0.0, cf = 0.0, tt = 0.0, scale = 0.0, ff = 0.0, d2 = 0.0, alph = 0.0;
+    if (valid) {
+      const double* o = pgeo + pq;
+      sf = o[0];
+      cf = o[N];
+      scale = o[2 * N];
+      ff = o[3 * N];
+      d2 = o[4 * N];
+      tt = P.tim[pq];
+      alph = P.alph[pq];
+    }
     body_vectors(v, cf, sf, tt, valid ? P.ex[pq] : 0.0, valid ? P.ey[pq] : 0.0, valid ? P.ez[pq] : 0.0, gg, db,
                  dl, dw);
     body_vectors(v, cf, sf, tt, valid ? P.sx[pq] : 0.0, valid ? P.sy[pq] : 0.0, valid ? P.sz[pq] : 0.0, gg,
@@ -536,124 +546,142 @@ struct P2 {
 };
 static_assert(P2<4>::GT == 4 && P2<5>::GT == 2, "chain count must match ps_tile_chains");
 
-// One tile of phase 2 for one wave: lane (ps, sl) handles column 16b + ps
-// of the tile's points 4r + sl, r = 0..3.
+// Phase 2 rows: lane (ps, sl) handles column 16b + ps of the points 4r + sl
+// (r = 0..3) of a tile.  The column's kind is fixed per lane and block: a
+// Jacobian column (q < mfit), the residual (q == mfit) or padding.
 template <int NB>
-struct P2Tile {
-  double dv[NB];  // column sums of the tile's curve
-  double rave;    // 1 / brightness sum of the curve
-  int p0, lp, t0; // first point of the curve, its length, first point of the tile in it
-  bool conv;      // the convexity tile (rows ndata .. ndata+2)
+struct P2Cols {
+  unsigned jm = 0, rm = 0;  // bit b: block b's column is a Jacobian column / the residual
+  __device__ __forceinline__ P2Cols(int ps, int mf) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int q = 16 * b + ps;
+      if (q < mf) jm |= 1u << b;
+      if (q == mf) rm |= 1u << b;
+    }
+  }
+};
 
+// The products of one row into the wave's block pairs.
+template <int NB, int NPW, int GB>
+__device__ __forceinline__ void p2_products(const double* X, const double* XW, int gb, v4d* acc) {
+  int idx = 0;
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int jb = 0; jb <= ib; ++jb) {
+      if (idx % GB == gb) acc[idx / GB] = mfma(X[ib], XW[jb], acc[idx / GB]);
+      ++idx;
+    }
+}
+
+// One wave's share of phase 2: its chain's data tiles, the next row's loads
+// issued before the current row's products, then (on the chain that owns
+// it) the convexity tile, which closes the chain.  Writes the chain's
+// partial matrix.
+template <int NB>
+__device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w) {
+  using SH = P2<NB>;
+  const int tid = tid_opaque(), lane = tid & 63, ps = lane & 15, sl = lane >> 4, w = tid >> 6;
+  const int gt = w / SH::GB, gb = w % SH::GB;
+  const int mf = P.mfit, C = P.ncoef, QS = P.qs;
+  const P2Cols<NB> K2(ps, mf);
+  v4d acc[SH::NPW];
+#pragma unroll
+  for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
   struct Row {
     double J[NB];
     double y, sig, obs, s2;
   };
-
-  __device__ __forceinline__ void load(const PsDevProblem& P, const Slab& S, int T) {
-    const int ps = lane_id() & 15;
-    const int QS = P.qs;
-    conv = T >= P.ntiles;
-    // the convexity tile reads curve 0's sums and point 0's observations
-    // (discarded), so every load stays in bounds without branches
-    const int c = conv ? 0 : P.tile_cv[T];
-    const int cp0 = P.curve_off[c];
-    p0 = conv ? P.ndata : cp0;
-    lp = conv ? 3 : P.curve_off[c + 1] - cp0;
-    t0 = conv ? 0 : 16 * (T - P.tile_off[c]);
-    const double* ds = S.dsum + (size_t)c * (QS + 1);
+  // ALLJ: blocks 0 .. NB-2 hold Jacobian columns only (16 (NB - 1) <= mfit,
+  // true for every input the triangulation and basis produce); only the
+  // last block then needs the per-lane column kind.
+  auto tiles = [&](auto allj_tag) __attribute__((always_inline)) {
+    constexpr bool ALLJ = decltype(allj_tag)::value;
+    for (int T = gt; T < P.ntiles; T += SH::GT) {
+      const int c = P.tile_cv[T];
+      const int cp0 = P.curve_off[c];
+      const int lp = P.curve_off[c + 1] - cp0;
+      const int t0 = 16 * (T - P.tile_off[c]);
+      const double* ds = S.dsum + (size_t)c * (QS + 1);
+      double dv[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) dv[b] = ds[16 * b + ps];
-    rave = ds[QS];
-  }
-
-  __device__ __forceinline__ Row row(const PsDevProblem& P, const Slab& S, int r) const {
-    const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
-    Row R;
-    const int pin = t0 + 4 * r + sl;
-    const int pt = p0 + (pin < lp ? pin : 0);
-    const int po = conv ? 0 : pt;
-    const double* jr = S.jbuf + (size_t)pt * P.qs;
+      for (int b = 0; b < NB; ++b) dv[b] = ds[16 * b + ps];
+      const double rave = ds[QS];
+      const double lpd = (double)lp;
+      auto load = [&](int r) __attribute__((always_inline)) {
+        Row R;
+        const int pin = t0 + 4 * r + sl;
+        const int pt = cp0 + (pin < lp ? pin : 0);
+        const double* jr = S.jbuf + (size_t)pt * QS + ps;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b + ps];
-    R.y = S.ybuf[pt];
-    R.sig = P.sig[po];
-    R.obs = P.bright[po];
-    R.s2 = P.s2w[po];
-    return R;
-  }
-
-  template <int NPW, int GB>
-  __device__ __forceinline__ void accumulate(const Row& R, int r, int mf, int C, double conw_w, int gb,
-                                             v4d* acc) const {
-    const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
-    const bool ok = t0 + 4 * r + sl < lp;
-    double coef = 0.0, coef1 = 0.0, res = 0.0, s2v = 0.0;
-    if (ok) {
-      if (conv) {
-        res = 0.0 - R.y;
-        s2v = conw_w * 1.0;
-      } else {
-        coef = (R.sig * (double)lp) * rave;
-        coef1 = R.y * rave;
-        const double ym = coef * R.y;
-        res = R.obs - ym;
-        s2v = R.s2;
-      }
+        for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b];
+        R.y = S.ybuf[pt];
+        R.sig = P.sig[pt];
+        R.obs = P.bright[pt];
+        R.s2 = P.s2w[pt];
+        return R;
+      };
+      // FULL: the tile has 16 points (every tile but a curve's last one)
+      auto rows = [&](auto full_tag) __attribute__((always_inline)) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        Row rw[2];
+        rw[0] = load(0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r < 3) rw[(r + 1) & 1] = load(r + 1);
+          const Row& cur = rw[r & 1];
+          const bool ok = FULL || t0 + 4 * r + sl < lp;
+          // relative renormalisation (Start.cu:2656-2715)
+          const double coef = (cur.sig * lpd) * rave;
+          const double coef1 = cur.y * rave;
+          const double res = cur.obs - coef * cur.y;
+          const double s2v = ok ? cur.s2 : 0.0;
+          double X[NB], XW[NB];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const double xj = coef * fma(-coef1, dv[b], cur.J[b]);
+            double xv;
+            if (ALLJ && b < NB - 1)
+              xv = xj;
+            else
+              xv = (K2.jm >> b) & 1u ? xj : (K2.rm >> b) & 1u ? res : 0.0;
+            xv = ok ? xv : 0.0;
+            X[b] = xv;
+            XW[b] = s2v * xv;
+          }
+          p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
+        }
+      };
+      if (t0 + 16 <= lp)
+        rows(std::true_type{});
+      else
+        rows(std::false_type{});
     }
+  };
+  if (16 * (NB - 1) <= mf)
+    tiles(std::true_type{});
+  else
+    tiles(std::false_type{});
+  // the convexity pseudo-curve (rows ndata .. ndata+2, Start.cu:2539-2612):
+  // tile T = ntiles, the last tile of chain ntiles mod GT; only its first
+  // row group holds rows (lanes sl < 3)
+  if (P.ntiles % SH::GT == gt) {
+    const bool ok = sl < 3;
+    const int pt = P.ndata + (ok ? sl : 0);
+    const double* jr = S.jbuf + (size_t)pt * QS + ps;
+    const double res = 0.0 - S.ybuf[pt];
+    const double s2v = ok ? conw_w * 1.0 : 0.0;
     double X[NB], XW[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int q = 16 * b + ps;
-      double xv = 0.0;
-      if (ok) {
-        if (q < mf) {
-          if (conv)
-            xv = (q + 2 <= C) ? R.J[b] : 0.0;
-          else
-            xv = coef * fma(-coef1, dv[b], R.J[b]);
-        } else if (q == mf) {
-          xv = res;
-        }
-      }
+      double xv = (K2.jm >> b) & 1u ? (q + 2 <= C ? jr[16 * b] : 0.0) : (K2.rm >> b) & 1u ? res : 0.0;
+      xv = ok ? xv : 0.0;
       X[b] = xv;
       XW[b] = s2v * xv;
     }
-    int idx = 0;
-#pragma unroll
-    for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-      for (int jb = 0; jb <= ib; ++jb) {
-        if (idx % GB == gb) acc[idx / GB] = mfma(X[ib], XW[jb], acc[idx / GB]);
-        ++idx;
-      }
-  }
-};
-
-// One wave's share of phase 2: the next row group's loads are issued
-// before the current one's products.  Writes the chain's partial matrix.
-template <int NB>
-__device__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w) {
-  using SH = P2<NB>;
-  using Row = typename P2Tile<NB>::Row;
-  const int tid = tid_opaque(), lane = tid & 63, ps = lane & 15, sl = lane >> 4, w = tid >> 6;
-  const int gt = w / SH::GB, gb = w % SH::GB;
-  const int mf = P.mfit, C = P.ncoef;
-  v4d acc[SH::NPW];
-#pragma unroll
-  for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
-  const int ntot = P.ntiles + 1;  // data tiles + the convexity tile
-  for (int T = gt; T < ntot; T += SH::GT) {
-    P2Tile<NB> tl;
-    tl.load(P, S, T);
-    Row cur = tl.row(P, S, 0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      Row nxt;
-      if (r < 3) nxt = tl.row(P, S, r + 1);
-      tl.template accumulate<SH::NPW, SH::GB>(cur, r, mf, C, conw_w, gb, acc);
-      if (r < 3) cur = nxt;
-    }
+    p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
   }
   double* pt = S.part + (size_t)gt * 256 * NB * NB;
   const int ld = 16 * NB;
@@ -674,7 +702,7 @@ __device__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w
 // beta in row mfit, chi^2 at [mfit][mfit]); facet areas of a in L.nrm[4f+3];
 // raw Jacobian rows of every point (and the convexity rows) in the slab.
 template <int NB, int NBS>
-__device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, const Slab& S,
+__device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, const Slab& S,
                          unsigned long long* vis_count) {
   using SH = P2<NB>;
   const int tid = tid_opaque(), w = tid >> 6, lane = tid & 63;
@@ -683,6 +711,29 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
   curv_block(P, a, L.nrm);
   if (tid == 0) store_inv(make_inv(P, a), L.inv);
   __syncthreads();
+  // per-point rotation and phase-function terms and the body-frame
+  // directions of Earth and Sun (Start.cu:2095-2195), once per evaluation
+  {
+    const Inv v = load_inv(L.inv);
+    for (int pt = tid; pt < N; pt += WG) {
+      const PointScalars q = point_scalars(P, v, pt);
+      double g[3], g0[3], scr[3];
+      body_vectors(v, q.cf, q.sf, q.tt, P.ex[pt], P.ey[pt], P.ez[pt], g, scr, scr, scr);
+      body_vectors(v, q.cf, q.sf, q.tt, P.sx[pt], P.sy[pt], P.sz[pt], g0, scr, scr, scr);
+      double* o = S.pgeo + pt;
+      o[0] = q.sf;
+      o[N] = q.cf;
+      o[2 * N] = q.scale;
+      o[3 * N] = q.ff;
+      o[4 * N] = q.d2;
+      o[5 * N] = g[0];
+      o[6 * N] = g[1];
+      o[7 * N] = g[2];
+      o[8 * N] = g0[0];
+      o[9 * N] = g0[1];
+      o[10 * N] = g0[2];
+    }
+  }
   PS_TMARK(0);
   // convexity pseudo-curve (Start.cu:2539-2612), rows N..N+2:
   // y_j = sum Area n_j, J_j[c] = sum (Area n_j) Y_c
@@ -698,14 +749,15 @@ __device__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, d
       S.ybuf[N + j] = ys;
     }
   }
+  __syncthreads();  // the per-point table is complete
   // phase 1 over every tile of every curve
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
   for (int T = w; T < P.ntiles; T += NW) {
     const int c = P.tile_cv[T];
     const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
-    nvis += phase1_tile<NB, NBS>(P, L, p0, lp, T - P.tile_off[c], S.jbuf, S.ybuf, S.tsum + (size_t)T * (QS + 1),
-                                 rowbuf);
+    nvis += phase1_tile<NB, NBS>(P, L, S.pgeo, p0, lp, T - P.tile_off[c], S.jbuf, S.ybuf,
+                                 S.tsum + (size_t)T * (QS + 1), rowbuf);
   }
   PS_TMARK(1);
   __syncthreads();

@@ -77,6 +77,8 @@ struct PsSlab {
   size_t dsum;   // [ncurves][qs+1] per-curve column sums (column qs: 1 / brightness sum)
   size_t part;   // [chains][16nb][16nb] partial augmented normal matrices
   size_t best;   // [K][K] accepted normal matrix
+  size_t pgeo;   // [11][ndata] per-point terms of the evaluated parameters: sin and cos of the
+                 // rotation angle, scale, ff, d2, body-frame Earth (3) and Sun (3) directions
   size_t stride;
 };
 
@@ -98,6 +100,8 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
   o += al((size_t)ps_tile_chains(P.nb) * 256 * P.nb * P.nb);
   s.best = o;
   o += al(K * K);
+  s.pgeo = o;
+  o += al((size_t)11 * P.ndata);
   s.stride = o;
   return s;
 }
