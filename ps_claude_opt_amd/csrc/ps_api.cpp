@@ -194,7 +194,9 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
 #define UP(dst, ptr, cnt) \
   if ((rc = upload(c, ptr, cnt, &dst)) != PS_OK) return rc
   {
-    std::vector<double> yl((size_t)F * 16 * nbs, 0.0);
+    // PS_FACET_PAD zero facets after the last one: the facet loop's
+    // prefetches run up to three 4-facet steps past the end
+    std::vector<double> yl((size_t)(F + PS_FACET_PAD) * 16 * nbs, 0.0);
     for (int f = 0; f < F; ++f)
       for (int q = 0; q <= C - 2; ++q)
         yl[((size_t)f * 16 + (q & 15)) * nbs + (q >> 4)] = p0->d_sphere[(size_t)(q + 1) * F + f];

@@ -228,7 +228,7 @@ __device__ __forceinline__ void point_geometry(const PsDevProblem& P, const Inv&
 // LDS layout of one workgroup (dynamic shared memory).
 struct Lds {
   double* work;    // [K*K] normal matrix / augmented Gauss-Jordan matrix
-  double* nrm;     // [F][4] facet normal x, y, z and the current facet area
+  double* nrm;     // [F + PS_FACET_PAD][4] facet normal x, y, z and the current facet area (padding: 0)
   const double* ytab;  // [F][16][nbs] shape basis (global table, read through L2)
   double* cgg;     // [ma]
   double* atry;    // [ma]
@@ -262,7 +262,7 @@ __device__ __forceinline__ Lds carve_lds(double* base, const PsDevProblem& P) {
   L.work = p;
   p += K * K;
   L.nrm = p;
-  p += 4 * P.nfac;
+  p += 4 * (P.nfac + PS_FACET_PAD);
   L.ytab = P.yl;
   L.cgg = p;
   p += P.ma;
@@ -279,10 +279,11 @@ __device__ __forceinline__ Lds carve_lds(double* base, const PsDevProblem& P) {
 // Copies the facet normals into the workgroup's LDS once per launch.
 __device__ __forceinline__ void stage_tables(const PsDevProblem& P, const Lds& L) {
   const int F = P.nfac;
-  for (int f = threadIdx.x; f < F; f += WG) {
-    L.nrm[4 * f + 0] = P.nx[f];
-    L.nrm[4 * f + 1] = P.ny[f];
-    L.nrm[4 * f + 2] = P.nz[f];
+  for (int f = threadIdx.x; f < F + PS_FACET_PAD; f += WG) {
+    const bool in = f < F;
+    L.nrm[4 * f + 0] = in ? P.nx[f] : 0.0;
+    L.nrm[4 * f + 1] = in ? P.ny[f] : 0.0;
+    L.nrm[4 * f + 2] = in ? P.nz[f] : 0.0;
     L.nrm[4 * f + 3] = 0.0;
   }
   __syncthreads();
@@ -373,17 +374,17 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   // register-rotation moves.
   const int NK = F / 4;
   const double* __restrict__ nr = L.nrm + 4 * sl;
-  const double* __restrict__ yb = L.ytab + (size_t)(16 * sl + ps) * NBS;
-  unsigned nvis = 0;
+  const double* __restrict__ ytab = L.ytab;  // uniform base; step k's rows at ytab + 64 NBS k
+  const int ly = (16 * sl + ps) * NBS;       // this lane's offset in a step's rows
+  unsigned nvis = 0;  // this lane's visible (point, facet) pairs
   auto step = [&](double n0, double n1, double n2, double ar, const double* y) {
     const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
     const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
     const bool vis = (mu > kTiny) && (mu0 > kTiny);
-    const unsigned long long bal = __ballot(vis);
-    if (bal != 0) {  // a whole step of invisible facets adds exact zeros
-      nvis += (unsigned)__popcll(bal);
+    if (__builtin_amdgcn_ballot_w64(vis) != 0) {  // a whole step of invisible facets adds exact zeros
       double w = 0.0;
       if (vis) {
+        ++nvis;
         const double rd = recip(mu + mu0);
         const double mm = mu * mu0;
         const double sc = mm * fma(cls, rd, cl);
@@ -406,30 +407,35 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
       for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, y[b], gat[b]);
     }
   };
+  // the prefetches of the last steps read up to three steps past the end:
+  // the zero padding of both tables (PS_FACET_PAD facets)
   double a0 = nr[0], a1 = nr[1], a2 = nr[2], aar = nr[3];
   double yA[NBS], yB[NBS];
 #pragma unroll
   for (int b = 0; b < NBS; ++b) {
-    yA[b] = yb[b];
-    yB[b] = yb[(NK > 1 ? (size_t)64 * NBS : 0) + b];
+    yA[b] = ytab[ly + b];
+    yB[b] = ytab[64 * NBS + ly + b];
   }
+  // Every load issued so far completes here, once per tile, so that the
+  // loop's own waits count only the prefetches in flight (otherwise the
+  // wait before the even step's MFMAs drains the odd step's prefetch too).
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   for (int k = 0; k < NK; k += 2) {
-    const int k1 = k + 1 < NK ? k + 1 : k;
-    const double* nb = nr + 16 * k1;
+    const double* nb = nr + 16 * (k + 1);
     const double b0 = nb[0], b1 = nb[1], b2 = nb[2], bar = nb[3];
     step(a0, a1, a2, aar, yA);
-    const int k2 = k + 2 < NK ? k + 2 : k;
+    const double* y2 = ytab + (size_t)(k + 2) * 64 * NBS;
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) yA[b] = yb[(size_t)k2 * 64 * NBS + b];
-    const double* na = nr + 16 * k2;
+    for (int b = 0; b < NBS; ++b) yA[b] = y2[ly + b];
+    const double* na = nr + 16 * (k + 2);
     a0 = na[0];
     a1 = na[1];
     a2 = na[2];
     aar = na[3];
     if (k + 1 < NK) step(b0, b1, b2, bar, yB);  // odd step count: F = 4 (mod 8)
-    const int k3 = k + 3 < NK ? k + 3 : k1;
+    const double* y3 = ytab + (size_t)(k + 3) * 64 * NBS;
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) yB[b] = yb[(size_t)k3 * 64 * NBS + b];
+    for (int b = 0; b < NBS; ++b) yB[b] = y3[ly + b];
   }
   br = xsum16_32(br);
   t4 = xsum16_32(t4);
@@ -493,29 +499,41 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
 #pragma unroll
   for (int b = 0; b < NB; ++b) tsq[b] = 0.0;
   double tsy = 0.0;
+  // Blocks below NBS - 1 hold shape columns only (NBS = ceil((C - 1) / 16));
+  // FULL: all 16 points of the tile exist (every tile but a curve's last)
+  auto rows_out = [&](auto full_tag) __attribute__((always_inline)) {
+    constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int ptl = 4 * r + sl;
-    const int prow = t * 16 + ptl;
-    const bool ok = prow < lp;
-    const double* rb = rowbuf + ptl * 16;
-    double* jr = jbuf + (size_t)(p0 + (ok ? prow : 0)) * QS;
+    for (int r = 0; r < 4; ++r) {
+      const int ptl = 4 * r + sl;
+      const int prow = t * 16 + ptl;
+      const bool ok = FULL || prow < lp;
+      const double* rb = rowbuf + ptl * 16;
+      double* jr = jbuf + (size_t)(p0 + (ok ? prow : 0)) * QS;
+      const double sc9 = rb[9];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int q = 16 * b + ps;
-      double val = 0.0;
-      if (q <= C - 2) {
-        if (b < NBS) val = rb[9] * gat[b < NBS ? b : 0][r];
-      } else if (q <= C + 6) {
-        val = rb[q - (C - 1)];
+      for (int b = 0; b < NB; ++b) {
+        const int q = 16 * b + ps;
+        double val;
+        if (b < NBS - 1) {
+          val = sc9 * gat[b][r];
+        } else {
+          const int k = q - (C - 1);  // non-shape column index, 0..7
+          const double ns = (k >= 0 && k <= 7) ? rb[k >= 0 && k <= 7 ? k : 0] : 0.0;
+          val = (b < NBS && q <= C - 2) ? sc9 * gat[b < NBS ? b : 0][r] : ns;
+        }
+        if (FULL || ok) jr[q] = val;
+        tsq[b] = tsq[b] + (FULL ? val : (ok ? val : 0.0));
       }
-      if (ok) jr[q] = val;
-      tsq[b] = tsq[b] + (ok ? val : 0.0);
+      const double yv = rb[8];
+      if (ok && ps == 0) ybuf[p0 + prow] = yv;
+      tsy = tsy + (FULL ? yv : (ok ? yv : 0.0));
     }
-    const double yv = rb[8];
-    if (ok && ps == 0) ybuf[p0 + prow] = yv;
-    tsy = tsy + (ok ? yv : 0.0);
-  }
+  };
+  if (t * 16 + 16 <= lp)
+    rows_out(std::true_type{});
+  else
+    rows_out(std::false_type{});
 #pragma unroll
   for (int b = 0; b < NB; ++b) tsq[b] = xsum16_32(tsq[b]);
   tsy = xsum16_32(tsy);
@@ -793,7 +811,11 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
     if (j > i) L.work[i * K + j] = L.work[j * K + i];
   }
   __syncthreads();
-  if (vis_count && lane == 0) atomicAdd(vis_count, nvis);
+  if (vis_count) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nvis += __shfl_xor(nvis, off);
+    if (lane == 0) atomicAdd(vis_count, nvis);
+  }
   PS_TMARK(6);
 }
 
@@ -1237,7 +1259,7 @@ __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2
 // ---------------------------------------------------------------------------
 size_t ps_fit_lds_bytes(const PsDevProblem& P) {
   const int K = P.mfit + 1;
-  const size_t d = (size_t)K * K + 4 * P.nfac + 2 * P.ma + NW * 256 + 2 * NW + 16;
+  const size_t d = (size_t)K * K + 4 * (P.nfac + PS_FACET_PAD) + 2 * P.ma + NW * 256 + 2 * NW + 16;
   return d * sizeof(double) + 64;
 }
 

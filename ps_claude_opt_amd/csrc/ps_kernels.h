@@ -6,6 +6,7 @@
 #include <cstddef>
 
 #define PS_OUT_STRIDE 10   // doubles per pair result
+#define PS_FACET_PAD 16    // zero facets after the last one in the basis table and the LDS normals
 
 // Device pointers and scalars of one prepared problem (passed by value).
 struct PsDevProblem {
@@ -26,7 +27,8 @@ struct PsDevProblem {
   int nbs;  // blocks holding shape columns
   const double *nx, *ny, *nz, *darea;
   const double *pleg, *fc, *fs;  // curv tables
-  const double* yl;              // [F][16][nbs]: shape column q = 16b + r (parameter q+2) at [f][r][b]
+  const double* yl;              // [F + PS_FACET_PAD][16][nbs]: shape column q = 16b + r (parameter q+2) at
+                                 // [f][r][b]; the padding facets are zero
   const double* cg_first;        // [ncoef]
   const int* cmap;               // [mfit] compact index -> parameter index (0-based)
   // model and LM controls
