@@ -235,12 +235,16 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
       alph[i] = std::acos(std::fmin(1.0, std::fmax(-1.0, dd)));
       s2w[i] = p->sigr2[i] * p->weight[i];
     }
-    std::vector<int> off(L + 1, 0), toff(L + 1, 0), tcv;
+    std::vector<int> off(L + 1, 0), toff(L + 1, 0), tcv, tinfo;
     for (int i = 0; i < L; ++i) {
       off[i + 1] = off[i] + p->l_points[i];
       const int nt = (p->l_points[i] + 15) / 16;
       toff[i + 1] = toff[i] + nt;
-      for (int t = 0; t < nt; ++t) tcv.push_back(i);
+      for (int t = 0; t < nt; ++t) {
+        tcv.push_back(i);
+        const int n_in = std::min(16, p->l_points[i] - 16 * t);
+        tinfo.insert(tinfo.end(), {off[i] + 16 * t, n_in, i, p->l_points[i]});
+      }
     }
     UP(d.tim, p->tim, N);
     UP(d.ex, p->ee, N);
@@ -256,6 +260,7 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
     UP(d.curve_off, off.data(), L + 1);
     UP(d.tile_off, toff.data(), L + 1);
     UP(d.tile_cv, tcv.data(), tcv.size());
+    UP(d.tile_info, tinfo.data(), tinfo.size());
     UP(d.cg_first, p->cg_first, C);
     d.ncurves = L;
     d.ndata = N;

@@ -617,10 +617,9 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
   auto tiles = [&](auto allj_tag) __attribute__((always_inline)) {
     constexpr bool ALLJ = decltype(allj_tag)::value;
     for (int T = gt; T < P.ntiles; T += SH::GT) {
-      const int c = P.tile_cv[T];
-      const int cp0 = P.curve_off[c];
-      const int lp = P.curve_off[c + 1] - cp0;
-      const int t0 = 16 * (T - P.tile_off[c]);
+      // first point, points in the tile, curve, curve length (one load)
+      const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
+      const int tp0 = ti.x, n_in = ti.y, c = ti.z, lp = ti.w;
       const double* ds = S.dsum + (size_t)c * (QS + 1);
       double dv[NB];
 #pragma unroll
@@ -629,8 +628,8 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
       const double lpd = (double)lp;
       auto load = [&](int r) __attribute__((always_inline)) {
         Row R;
-        const int pin = t0 + 4 * r + sl;
-        const int pt = cp0 + (pin < lp ? pin : 0);
+        const int pin = 4 * r + sl;
+        const int pt = tp0 + (pin < n_in ? pin : 0);
         const double* jr = S.jbuf + (size_t)pt * QS + ps;
 #pragma unroll
         for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b];
@@ -649,7 +648,7 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
         for (int r = 0; r < 4; ++r) {
           if (r < 3) rw[(r + 1) & 1] = load(r + 1);
           const Row& cur = rw[r & 1];
-          const bool ok = FULL || t0 + 4 * r + sl < lp;
+          const bool ok = FULL || 4 * r + sl < n_in;
           // relative renormalisation (Start.cu:2656-2715)
           const double coef = (cur.sig * lpd) * rave;
           const double coef1 = cur.y * rave;
@@ -671,7 +670,7 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
           p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
         }
       };
-      if (t0 + 16 <= lp)
+      if (n_in == 16)
         rows(std::true_type{});
       else
         rows(std::false_type{});
@@ -723,7 +722,8 @@ template <int NB, int NBS>
 __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, const Slab& S,
                          unsigned long long* vis_count) {
   using SH = P2<NB>;
-  const int tid = tid_opaque(), w = tid >> 6, lane = tid & 63;
+  const int tid = tid_opaque(), lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, in a scalar register
   const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac, N = P.ndata;
   PS_TSTART();
   curv_block(P, a, L.nrm);
@@ -772,10 +772,10 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
   for (int T = w; T < P.ntiles; T += NW) {
-    const int c = P.tile_cv[T];
-    const int p0 = P.curve_off[c], lp = P.curve_off[c + 1] - p0;
-    nvis += phase1_tile<NB, NBS>(P, L, S.pgeo, p0, lp, T - P.tile_off[c], S.jbuf, S.ybuf,
-                                 S.tsum + (size_t)T * (QS + 1), rowbuf);
+    // the tile's first point and its number of points (one load)
+    const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
+    nvis += phase1_tile<NB, NBS>(P, L, S.pgeo, ti.x, ti.y, 0, S.jbuf, S.ybuf, S.tsum + (size_t)T * (QS + 1),
+                                 rowbuf);
   }
   PS_TMARK(1);
   __syncthreads();

@@ -19,6 +19,7 @@ struct PsDevProblem {
   // 16-point tiles of the curves (tiles never straddle curves)
   const int* tile_off;  // [ncurves+1] first tile of each curve
   const int* tile_cv;   // [ntiles] curve of each tile
+  const int* tile_info; // [ntiles][4] first point, points in the tile (1..16), curve, curve length
   int ntiles;
   // facets and basis
   int nfac, lmax, mmax, ncoef, ma, mfit;
