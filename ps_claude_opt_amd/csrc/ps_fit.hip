@@ -205,26 +205,6 @@ __device__ __forceinline__ void body_vectors(const Inv& v, double cf, double sf,
   dw[2] = 0.0;
 }
 
-struct Geo {
-  double g[3], g0[3], db[3], db0[3], dl[3], dl0[3], dw[3], dw0[3];
-  double scale, ff, d2, alph;
-};
-
-__device__ __forceinline__ void point_geometry(const PsDevProblem& P, const Inv& v, int pt, Geo& G) {
-  const double t = P.tim[pt];
-  double f = fma(v.omega, t, P.phi0);
-  f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
-  double sf, cf;
-  psm::sincos(f, &sf, &cf);
-  const double alph = P.alph[pt];
-  G.alph = alph;
-  G.ff = psm::exp2(-1.44269504088896 * (alph * v.invD));
-  G.scale = fma(v.kph, alph, fma(v.A, G.ff, 1.0));
-  G.d2 = ((v.A * G.ff) * alph) * v.invD2;
-  body_vectors(v, cf, sf, t, P.ex[pt], P.ey[pt], P.ez[pt], G.g, G.db, G.dl, G.dw);
-  body_vectors(v, cf, sf, t, P.sx[pt], P.sy[pt], P.sz[pt], G.g0, G.db0, G.dl0, G.dw0);
-}
-
 // LDS layout of one workgroup (dynamic shared memory).
 struct Lds {
   double* work;    // [K*K] normal matrix / augmented Gauss-Jordan matrix
@@ -368,10 +348,9 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   // shape columns 16b + ps at ytab[(16f + ps) * NBS + b].  Software
   // pipeline over two register sets (A: even steps, B: odd steps; the
   // triangulation's step count F / 4 = 2 nrows^2 is even): the normals of a
-  // step are read from
-  // LDS one step ahead, its basis values from L2 as soon as the step two
-  // before has issued its MFMAs.  Two sets keep the pipeline free of
-  // register-rotation moves.
+  // step are read from LDS one step ahead, its basis values from L2 as soon
+  // as the step two before has issued its MFMAs.  Two sets keep the
+  // pipeline free of register-rotation moves.
   const int NK = F / 4;
   const double* __restrict__ nr = L.nrm + 4 * sl;
   const double* __restrict__ ytab = L.ytab;  // uniform base; step k's rows at ytab + 64 NBS k
