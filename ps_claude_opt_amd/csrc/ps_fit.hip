@@ -212,7 +212,7 @@ struct Lds {
   const double* ytab;  // [F][16][nbs] shape basis (global table, read through L2)
   double* cgg;     // [ma]
   double* atry;    // [ma]
-  double* rowbuf;  // [NW][16][16] per-point non-shape values
+  double* rowbuf;  // [NW][16][16] per-point non-shape values of phase 1 (aliases work)
   double* red;     // [2*NW] reduction scratch
   double* inv;     // [16] pair invariants of the evaluated parameters
 };
@@ -239,8 +239,12 @@ __device__ __forceinline__ Lds carve_lds(double* base, const PsDevProblem& P) {
   Lds L;
   const int K = P.mfit + 1;
   double* p = base;
+  // the tile rows of phase 1 live in the normal-matrix area: during the
+  // facet pass the matrix is dead (the Gauss-Jordan solution has been
+  // consumed; the evaluation writes the new matrix after its last barrier)
   L.work = p;
-  p += K * K;
+  L.rowbuf = p;
+  p += K * K > NW * 256 ? K * K : NW * 256;
   L.nrm = p;
   p += 4 * (P.nfac + PS_FACET_PAD);
   L.ytab = P.yl;
@@ -248,8 +252,6 @@ __device__ __forceinline__ Lds carve_lds(double* base, const PsDevProblem& P) {
   p += P.ma;
   L.atry = p;
   p += P.ma;
-  L.rowbuf = p;
-  p += NW * 256;
   L.red = p;
   p += 2 * NW;
   L.inv = p;
@@ -861,6 +863,7 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       u0 |= 1ull << icol;
     else
       u1 |= 1ull << (icol - 64);
+
     // rows irow and icol before anyone rewrites them
     double pv0 = 0.0, pv1 = 0.0, cv0 = 0.0, cv1 = 0.0;
     if (h0) {
@@ -1238,7 +1241,8 @@ __global__ void ps_math_kernel(int n, const double* x, double* out, double* out2
 // ---------------------------------------------------------------------------
 size_t ps_fit_lds_bytes(const PsDevProblem& P) {
   const int K = P.mfit + 1;
-  const size_t d = (size_t)K * K + 4 * (P.nfac + PS_FACET_PAD) + 2 * P.ma + NW * 256 + 2 * NW + 16;
+  const size_t d = (size_t)(K * K > NW * 256 ? K * K : NW * 256) + 4 * (P.nfac + PS_FACET_PAD) + 2 * P.ma +
+                   2 * NW + 16;
   return d * sizeof(double) + 64;
 }
 
