@@ -812,7 +812,8 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
 // TWO: columns beyond 64 exist (K > 64).
 template <bool TWO>
 __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* red) {
-  const int tid = tid_opaque(), lane = tid & 63, w = tid >> 6;
+  const int tid = tid_opaque(), lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, in a scalar register
   const int c0 = lane, c1 = lane + 64;
   const bool h0 = c0 < K, h1 = TWO && c1 < K;
   unsigned long long u0 = 0, u1 = 0;  // pivoted rows / columns
@@ -863,7 +864,9 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       u0 |= 1ull << icol;
     else
       u1 |= 1ull << (icol - 64);
-
+    // this lane's columns still open for the next pivot
+    const bool open0 = c0 < mf && !used(c0);
+    const bool open1 = TWO && c1 < mf && !used(c1);
     // rows irow and icol before anyone rewrites them
     double pv0 = 0.0, pv1 = 0.0, cv0 = 0.0, cv1 = 0.0;
     if (h0) {
@@ -922,13 +925,13 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
           const double base = (c0 == icol) ? 0.0 : (sw ? cv0 : b0[u]);
           const double nv = fma(-pn0, dum, base);
           a[ll * K + c0] = nv;
-          if (cand && c0 < mf && !used(c0)) consider(nv, ll * 128 + c0);
+          if (cand && open0) consider(nv, ll * 128 + c0);
         }
         if (h1) {
           const double base = (c1 == icol) ? 0.0 : (sw ? cv1 : b1[u]);
           const double nv = fma(-pn1, dum, base);
           a[ll * K + c1] = nv;
-          if (cand && c1 < mf && !used(c1)) consider(nv, ll * 128 + c1);
+          if (cand && open1) consider(nv, ll * 128 + c1);
         }
       }
     }
