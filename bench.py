@@ -17,7 +17,7 @@ flops of the fitted pairs / kernel time measured with HIP events on the
 library's stream) and a CPU baseline: the repository's CPU FP64 oracle on a
 seeded sample of pairs of the same workload, on the host's cores.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--cpu-pairs 32] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--cpu-pairs 128] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -70,7 +70,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-configuration autotuner")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
-    ap.add_argument("--warm-iters", type=int, default=10, help="LM iterations per pair of the warm-started scan")
+    ap.add_argument("--warm-iters", type=int, default=20, help="LM iterations per pair of the warm-started scan")
+    ap.add_argument("--warm-refit", type=int, default=50,
+                    help="frequencies of the warm-started scan refitted with full cold fits")
     ap.add_argument("--work-dir", default=os.path.join(ROOT, "gpurun_out", "bench"))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates HBM byte counters)")
@@ -184,7 +186,7 @@ def main() -> int:
                       "truth_lambda": truth["lambda_deg"], "truth_beta": truth["beta_deg"]},
     }
     if not args.no_warm and n_max == info["n_max"] and os.path.exists(WARM_WEIGHTS):
-        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters)
+        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters, args.warm_refit)
     scan.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(path, conw_r, n_max, args.cpu_pairs)
@@ -192,24 +194,26 @@ def main() -> int:
     return 0
 
 
-def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int) -> dict:
+def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: int) -> dict:
     """The learned warm start on the same scan: every pair runs n_iter LM
-    iterations from the network's prediction, then the 10 best frequencies
-    are refitted cold (the plain scan's own 50-iteration fits), which decide
-    the reported winner.  The same truncated scan from the reference's start
-    is timed beside it (what the network adds), and both winners are compared
-    with the plain scan's (the same printed lambda and beta, period within
-    1e-5 relative, rms within 5e-4, chi2 within 0.5)."""
+    iterations from the network's prediction, then the `refit` best
+    frequencies are refitted cold (the plain scan's own 50-iteration fits),
+    which decide the reported winner.  The same truncated scan from the
+    reference's start is timed beside it (what the network adds), and both
+    winners are compared with the plain scan's (the same printed lambda and
+    beta, period within 1e-5 relative, rms within 5e-4, chi2 within 0.5).
+    How often each finds the plain winner on held-out asteroids is measured
+    by ps_claude_opt_amd/warm_eval.py."""
     import numpy as np
 
     from ps_claude_opt_amd import warm
 
     ws = warm.WarmStart.load(WARM_WEIGHTS)
     net = ws.network(scan)
-    out = {"n_iter": n_iter, "refit_frequencies": 10, "weights": os.path.relpath(WARM_WEIGHTS, ROOT)}
+    out = {"n_iter": n_iter, "refit_frequencies": refit, "weights": os.path.relpath(WARM_WEIGHTS, ROOT)}
     for name, w in (("warm", ws), ("truncated_cold", None)):
         t0 = time.perf_counter()
-        res, refit = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=10)
+        res, _ = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=refit)
         dt = time.perf_counter() - t0
         b = warm.best_line(res, 1, n_max)
         b["chi2"] = b["rms"] * b["rms"] * scan.info["ndata"]  # the output line's chi2 (start_CUDA.cu:1222-1226)
