@@ -213,9 +213,9 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
     out = {"n_iter": n_iter, "refit_frequencies": refit, "weights": os.path.relpath(WARM_WEIGHTS, ROOT)}
     for name, w in (("warm", ws), ("truncated_cold", None)):
         t0 = time.perf_counter()
-        res, _ = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=refit)
+        res, refitted = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=refit)
         dt = time.perf_counter() - t0
-        b = warm.best_line(res, 1, n_max)
+        b = warm.best_line(res, 1, n_max, among=refitted)
         b["chi2"] = b["rms"] * b["rms"] * scan.info["ndata"]  # the output line's chi2 (start_CUDA.cu:1222-1226)
         agree = (round(b["lambda_deg"]) == round(plain_best.lambda_deg) and round(b["beta_deg"]) == round(plain_best.beta_deg)
                  and abs(b["period_h"] - plain_best.period_h) <= 1e-5 * plain_best.period_h

@@ -273,7 +273,10 @@ def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, 
     then the `refit` frequencies with the lowest rms are fitted again cold,
     with the problem's own stop condition (all 10 poles), and those cold
     results replace the truncated ones.  Returns (results [n_count*10]
-    frequency-major, refitted frequency indices)."""
+    frequency-major, refitted frequency indices).  The scan's winner is the
+    best of the refitted frequencies (best_line(..., among=refitted)):
+    those are plain fits, and a warm-started truncated fit may end below a
+    converged cold one."""
     n = np.repeat(np.arange(n_start, n_start + n_count), 10)
     m = np.tile(np.arange(1, 11), n_count)
     p = np.zeros(n.size, np.int32)
@@ -292,12 +295,19 @@ def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, 
     return res, n_start + fr
 
 
-def best_line(res: np.ndarray, n_start: int, n_count: int) -> dict:
-    """The overall best (lowest rms) line of a scan's results."""
+def best_line(res: np.ndarray, n_start: int, n_count: int, among=None) -> dict:
+    """The overall best (lowest rms) line of a scan's results; `among`
+    (frequency indices, n_start-based) restricts the choice to those
+    frequencies, e.g. the cold refits of a warm scan."""
     best = best_lines(res, n_count)
     rows = np.arange(n_count) * 10 + best
     rms = res["dev"][rows]
-    f = int(np.nanargmin(np.where(np.isnan(rms), np.inf, rms)))
+    rms = np.where(np.isnan(rms), np.inf, rms)
+    if among is not None:
+        keep = np.zeros(n_count, bool)
+        keep[np.asarray(among, np.int64) - n_start] = True
+        rms = np.where(keep, rms, np.inf)
+    f = int(np.argmin(rms))
     r = res[rows[f]]
     return dict(freq_index=n_start + f, pole=int(best[f]) + 1, period_h=24.0 * float(r["period"]),
                 rms=float(r["dev"]), lambda_deg=float(r["lambda_deg"]), beta_deg=float(r["beta_deg"]),

@@ -14,9 +14,8 @@ report records where the plain winner ranks among the frequencies ordered by
 their best truncated rms.
 
 A warm scan that refits its `refit` best-ranked frequencies with full cold
-fits reproduces the plain winner exactly when that rank is below `refit`:
-truncated fits never reach a lower rms than the converged ones, and the
-refits are the plain fits themselves.
+fits, and takes its winner among those refits (plain fits, bit for bit),
+reproduces the plain winner exactly when that rank is below `refit`.
 """
 from __future__ import annotations
 

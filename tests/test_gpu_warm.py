@@ -219,3 +219,35 @@ def test_training_pipeline_generalises(tmp_path):
     gap_cold = np.nanmedian(np.abs(few_cold["dev"] - full["dev"])[ok])
     gap_warm = np.nanmedian(np.abs(few_warm["dev"] - full["dev"])[ok])
     assert gap_warm < 1.25 * gap_cold, (gap_warm, gap_cold)
+
+
+def test_warm_scan_refits_are_plain_fits(inputs):
+    """A warm scan's refitted frequencies carry the plain fits bit for bit,
+    and the warm winner (the best refit) is the plain winner whenever the
+    plain winner's frequency was refitted."""
+    import ps_claude_opt_amd as ps
+    from ps_claude_opt_amd import warm
+
+    path, _ = inputs("c1")
+    with ps.Scan(0) as s:
+        s.prepare_file(path)
+        n_max = min(s.info["n_max"], 80)
+        cw = 0.5
+        n = np.repeat(np.arange(1, n_max + 1), 10)
+        m = np.tile(np.arange(1, 11), n_max)
+        p = np.zeros(n.size, np.int32)
+        plain, _ = s.fit_items_from(p, n, m, cw, cw, None)
+        ws = warm.WarmStart.load(warm.DEFAULT_WEIGHTS)
+        net = ws.network(s)
+        res, fr = warm.warm_scan(s, ws, net, cw, 1, n_max, n_iter=4, refit=12)
+        net.close()
+    assert fr.size == 12
+    rows = ((fr - 1)[:, None] * 10 + np.arange(10)).ravel()
+    for f in ("dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "niter"):
+        assert np.array_equal(res[f][rows], plain[f][rows], equal_nan=True), f
+    p_best = warm.best_line(plain, 1, n_max)
+    w_best = warm.best_line(res, 1, n_max, among=fr)
+    if p_best["freq_index"] in set(fr.tolist()):
+        assert w_best == p_best
+    else:
+        assert w_best["rms"] >= p_best["rms"]
