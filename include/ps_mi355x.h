@@ -266,6 +266,16 @@ int ps_mlp_free(ps_mlp* net);
 int ps_set_launch(ps_ctx* ctx, int waves_per_pair, int wg_per_cu);
 /* The configuration the next launch will use (after ps_prepare*). */
 int ps_get_launch(const ps_ctx* ctx, int* waves_per_pair, int* wg_per_cu);
+/* HIP graphs (no counterpart in the reference, which launches each kernel
+ * from the host; for callers whose scan steps are short enough that launch
+ * overhead shows): with on != 0, every fit launch of this context (the
+ * work-counter reset and the persistent LM kernel) is captured once into a
+ * HIP graph and replayed while its arguments, grid and variant stay the
+ * same; item and result copies stay outside the graph.  Results are
+ * bit-identical with graphs on and off.  Default off. */
+int ps_set_graphs(ps_ctx* ctx, int on);
+/* Graph captures and replays so far on this context (either may be NULL). */
+int ps_graph_stats(const ps_ctx* ctx, int* captures, int* replays);
 /* Times every configuration on the given work items (as ps_fit_items with
  * conw_r_dev = conw_r), keeps the fastest for later calls on this context
  * and reports it with its fit-kernel time in milliseconds. */

@@ -249,6 +249,17 @@ class Scan:
         self._check(self._lib.ps_get_launch(self._h, C.byref(w), C.byref(g)))
         return dict(waves_per_pair=w.value, wg_per_cu=g.value)
 
+    def set_graphs(self, on: bool = True):
+        """Capture each fit launch into a HIP graph and replay it while the
+        launch arguments stay the same (results are bit-identical)."""
+        self._check(self._lib.ps_set_graphs(self._h, 1 if on else 0))
+
+    def graph_stats(self) -> dict:
+        c = C.c_int(0)
+        r = C.c_int(0)
+        self._check(self._lib.ps_graph_stats(self._h, C.byref(c), C.byref(r)))
+        return dict(captures=c.value, replays=r.value)
+
     def autotune(self, prob, n_list, m_list, conw_r) -> dict:
         """Times every launch configuration on these items and keeps the
         fastest (results do not depend on the configuration)."""

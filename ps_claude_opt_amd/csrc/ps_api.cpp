@@ -59,7 +59,56 @@ int upload(ps_ctx* c, const T* host, size_t n, const T** dev) {
   return PS_OK;
 }
 
+// Work-queue reset inside a captured scan step (a kernel, not a memset node).
+__global__ void ps_reset_counter_kernel(int* counter) {
+  if (threadIdx.x == 0) *counter = 0;
+}
+
+void drop_graph(ps_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+}
+
+// The scan step on the context's stream: reset of the work counter, then
+// the persistent fit launch -- eagerly, or as a captured HIP graph that is
+// replayed while the launch arguments, grid and variant stay the same.
+hipError_t launch_step(ps_ctx* c, const PsFitArgs& A, int grid, bool w8) {
+  if (!c->use_graphs) {
+    const int zero = 0;
+    hipError_t e = hipMemcpyAsync(c->counter, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return e;
+    return w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream);
+  }
+  if (!c->gexec || grid != c->ggrid || w8 != c->gw8 || std::memcmp(&A, &c->gargs, sizeof(A)) != 0) {
+    drop_graph(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ps_reset_counter_kernel, dim3(1), dim3(64), 0, c->stream, c->counter);
+    hipError_t el = hipGetLastError();
+    if (el == hipSuccess) el = w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream);
+    e = hipStreamEndCapture(c->stream, &g);
+    if (el != hipSuccess || e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return el != hipSuccess ? el : e;
+    }
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      return e;
+    }
+    std::memcpy(&c->gargs, &A, sizeof(A));
+    c->ggrid = grid;
+    c->gw8 = w8;
+    ++c->graph_captures;
+  }
+  ++c->graph_replays;
+  return hipGraphLaunch(c->gexec, c->stream);
+}
+
 void free_problem(ps_ctx* c) {
+  drop_graph(c);
   for (void* p : c->dbufs) hipFree(p);
   c->dbufs.clear();
   c->d_probs = nullptr;
@@ -97,7 +146,8 @@ int ensure_batch(ps_ctx* c, int n_items, int grid) {
 }
 
 PsFitArgs make_args(ps_ctx* c, int n_items) {
-  PsFitArgs A{};
+  PsFitArgs A;
+  std::memset(&A, 0, sizeof(A));  // padding too: graph replays compare argument blocks bytewise
   A.P0 = c->dps[0];
   A.probs = c->d_probs;
   A.item_p = c->d_items_p;
@@ -404,14 +454,12 @@ int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const 
       std::memcpy(&a0p[(size_t)k * ma], a0 + (size_t)order[k] * ma, ma * sizeof(double));
     PS_HIP(c, hipMemcpyAsync(c->d_a0, a0p.data(), a0p.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   }
-  const int zero = 0;
-  PS_HIP(c, hipMemcpyAsync(c->counter, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
   PsFitArgs A = make_args(c, n_items);
   A.item_a0 = a0 ? c->d_a0 : nullptr;
   A.out_a = a_final ? c->d_afinal : nullptr;
   A.n_iter_fixed = n_iter_fixed;
   PS_HIP(c, hipEventRecord(c->ev0, c->stream));
-  PS_HIP(c, w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream));
+  PS_HIP(c, launch_step(c, A, grid, w8));
   PS_HIP(c, hipEventRecord(c->ev1, c->stream));
   std::vector<double> res((size_t)n_items * PS_OUT_STRIDE);
   PS_HIP(c, hipMemcpyAsync(res.data(), c->d_out, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -910,6 +958,20 @@ int ps_set_launch(ps_ctx* c, int waves_per_pair, int wg_per_cu) {
   if (!c || (waves_per_pair != 4 && waves_per_pair != 8) || wg_per_cu < 0) return PS_E_ARG;
   c->waves = waves_per_pair;
   c->wg_per_cu = wg_per_cu;
+  return PS_OK;
+}
+
+int ps_set_graphs(ps_ctx* c, int on) {
+  if (!c) return PS_E_ARG;
+  c->use_graphs = on != 0;
+  if (!c->use_graphs) drop_graph(c);
+  return PS_OK;
+}
+
+int ps_graph_stats(const ps_ctx* c, int* captures, int* replays) {
+  if (!c) return PS_E_ARG;
+  if (captures) *captures = c->graph_captures;
+  if (replays) *replays = c->graph_replays;
   return PS_OK;
 }
 

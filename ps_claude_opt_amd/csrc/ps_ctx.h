@@ -61,6 +61,14 @@ struct ps_ctx {
   double* d_a0 = nullptr;      // [items][ma] initial parameters (warm starts)
   double* d_afinal = nullptr;  // [items][ma] final parameters
   size_t a_cap = 0;
+  // HIP graph of a scan step (work-counter reset + persistent fit launch),
+  // replayed while the launch arguments stay the same
+  bool use_graphs = false;
+  hipGraphExec_t gexec = nullptr;
+  PsFitArgs gargs{};
+  int ggrid = 0;
+  bool gw8 = false;
+  int graph_captures = 0, graph_replays = 0;
   // statistics of the last fit call
   double last_ms = 0;
   int last_launches = 0;
