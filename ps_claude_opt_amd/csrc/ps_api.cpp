@@ -480,7 +480,7 @@ int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const 
     unsigned long long cyc[16];
     if ((w8 ? psw8::ps_phase_cycles(cyc, true) : psw4::ps_phase_cycles(cyc, true)) == hipSuccess) {
       std::fprintf(stderr, "phase cycles (thread 0 of each workgroup, summed):");
-      for (int i = 0; i < 9; ++i) std::fprintf(stderr, " %d:%.4g", i, (double)cyc[i]);
+      for (int i = 0; i < 10; ++i) std::fprintf(stderr, " %d:%.4g", i, (double)cyc[i]);
       std::fprintf(stderr, "\n");
     }
   }

@@ -5,15 +5,21 @@
 //
 // Per mrqcof evaluation (reference Start.cu:2262-2828):
 //   curv          facet areas Area_f = Darea_f exp(sum P_l^m (c cos + s sin))
-//   phase 1       over all 16-point tiles of all curves, round-robin over
-//                 the waves; per 4-facet step lane l owns point (l & 15) and
-//                 facet (4k + (l >> 4)); it evaluates the scattering law and
-//                 the derivative sums on the VALU and feeds the facet weight
-//                 w = Area * S as the A operand of v_mfma_f64_16x16x4f64
-//                 against the basis table (B operand, read through L2),
-//                 which yields the shape Jacobian of 16 points x 16
-//                 coefficients per block.  Raw Jacobian rows and per-tile
-//                 column sums go to the workgroup's scratch slab.
+//   points        per-point record (rotation angle, pole-frame Earth and Sun
+//                 directions, phase-function terms) and a counting sort of
+//                 the points by view-direction cell
+//   phase 1       over 16-point tiles of the sorted points, claimed by the
+//                 waves one at a time; per 4-facet step lane l owns point
+//                 (l & 15) and facet (4k + (l >> 4)); it evaluates the
+//                 scattering law and the derivative sums on the VALU and
+//                 feeds the facet weight w = Area * S as the A operand of
+//                 v_mfma_f64_16x16x4f64 against the basis table (B operand,
+//                 read through L2), which yields the shape Jacobian of 16
+//                 points x 16 coefficients per block.  Steps in which no
+//                 point of the tile sees any of the 4 facets are skipped;
+//                 sorting by view direction makes such steps common.  Raw
+//                 Jacobian rows go to the workgroup's scratch slab.
+//   tile sums     column sums of each 16-point tile of a curve, from the rows
 //   curve sums    column sums d and brightness sum L of each curve, from
 //                 its tile sums
 //   phase 2       relative renormalisation J' = c (J - y d / L), residual
@@ -182,27 +188,78 @@ __device__ __forceinline__ Inv make_inv(const PsDevProblem& P, const double* a) 
 
 // Body-frame direction g = Rz(phi) P(beta') Rz(lambda) e and its
 // derivatives with respect to beta', lambda and omega.
-__device__ __forceinline__ void body_vectors(const Inv& v, double cf, double sf, double t, double ex, double ey,
-                                             double ez, double* g, double* db, double* dl, double* dw) {
-  const double ux = fma(v.sl, ey, v.clam * ex);
-  const double uy = fma(-v.sl, ex, v.clam * ey);
-  const double uz = ez;
-  const double vx = fma(-v.sb, uz, v.cb * ux);
-  const double vy = uy;
-  const double vz = fma(v.sb, ux, v.cb * uz);
-  g[0] = fma(sf, vy, cf * vx);
-  g[1] = fma(-sf, vx, cf * vy);
+// An ecliptic direction e (Earth or Sun) in the pole frame: u = e turned
+// by lambda about z, then v = u turned by beta about y (v_y = u_y).
+struct Frame {
+  double ux, uy, vx, vz;
+};
+
+__device__ __forceinline__ Frame pole_frame(const Inv& v, double ex, double ey, double ez) {
+  Frame r;
+  r.ux = fma(v.sl, ey, v.clam * ex);
+  r.uy = fma(-v.sl, ex, v.clam * ey);
+  r.vx = fma(-v.sb, ez, v.cb * r.ux);
+  r.vz = fma(v.sb, r.ux, v.cb * ez);
+  return r;
+}
+
+// Body-frame direction g at rotation angle (cos cf, sin sf).
+__device__ __forceinline__ void body_dir(double cf, double sf, double vx, double uy, double vz, double* g) {
+  g[0] = fma(sf, uy, cf * vx);
+  g[1] = fma(-sf, vx, cf * uy);
   g[2] = vz;
-  db[0] = -(cf * vz);
-  db[1] = sf * vz;
-  db[2] = vx;
-  const double w = v.cb * uy;
-  dl[0] = fma(cf, w, -(sf * ux));
-  dl[1] = fma(-sf, w, -(cf * ux));
-  dl[2] = v.sb * uy;
+}
+
+// Derivatives of g with respect to beta, lambda and the rotation rate.
+__device__ __forceinline__ void body_derivs(const Inv& v, double cf, double sf, double t, const Frame& r,
+                                            const double* g, double* db, double* dl, double* dw) {
+  db[0] = -(cf * r.vz);
+  db[1] = sf * r.vz;
+  db[2] = r.vx;
+  const double w = v.cb * r.uy;
+  dl[0] = fma(cf, w, -(sf * r.ux));
+  dl[1] = fma(-sf, w, -(cf * r.ux));
+  dl[2] = v.sb * r.uy;
   dw[0] = t * g[1];
   dw[1] = -(t * g[0]);
   dw[2] = 0.0;
+}
+
+// Fields of the per-point record (PS_REC doubles, one 128-byte line): the
+// rotation angle, the pole-frame Earth and Sun directions, the epoch and the
+// phase-function terms of one evaluation.
+enum : int {
+  RC_SF = 0, RC_CF, RC_VX, RC_UY, RC_VZ, RC_UX, RC_VX0, RC_UY0, RC_VZ0, RC_UX0,
+  RC_TT, RC_SCALE, RC_FF, RC_D2, RC_ALPH
+};
+static_assert(RC_ALPH < PS_REC, "point record overflows its line");
+
+// View-direction cell of a point: the cube-map cell (6 faces x 6 x 6,
+// rows traversed alternately forwards and backwards) of g + g0.  Points of
+// one cell see nearly the same facets.  Only the speed of the facet pass
+// depends on it, so it is computed in single precision.
+constexpr int kCellSide = 6;
+constexpr int kCells = 6 * kCellSide * kCellSide;
+static_assert(kCells <= 256, "cells are stored as bytes");
+
+__device__ __forceinline__ int view_cell(const double* g, const double* g0) {
+  const float h0 = (float)(g[0] + g0[0]), h1 = (float)(g[1] + g0[1]), h2 = (float)(g[2] + g0[2]);
+  const float a0 = fabsf(h0), a1 = fabsf(h1), a2 = fabsf(h2);
+  int ax;
+  float hm, u, w;
+  if (a0 >= a1 && a0 >= a2) {
+    ax = 0, hm = h0, u = h1, w = h2;
+  } else if (a1 >= a2) {
+    ax = 1, hm = h1, u = h2, w = h0;
+  } else {
+    ax = 2, hm = h2, u = h0, w = h1;
+  }
+  const float r = 0.5f * kCellSide * __builtin_amdgcn_rcpf(fmaxf(fabsf(hm), 1e-30f));
+  const float hi = kCellSide - 1;
+  int iu = (int)fminf(fmaxf(fmaf(u, r, 0.5f * kCellSide), 0.0f), hi);
+  const int iw = (int)fminf(fmaxf(fmaf(w, r, 0.5f * kCellSide), 0.0f), hi);
+  if (iw & 1) iu = kCellSide - 1 - iu;
+  return ((2 * ax + (hm > 0.0f ? 1 : 0)) * kCellSide + iw) * kCellSide + iu;
 }
 
 // LDS layout of one workgroup (dynamic shared memory).
@@ -219,7 +276,9 @@ struct Lds {
 
 // Pointers into one workgroup's scratch slab.
 struct Slab {
-  double *jbuf, *ybuf, *tsum, *dsum, *part, *best, *pgeo;
+  double *jbuf, *ybuf, *tsum, *dsum, *part, *best, *prec;
+  int* perm;
+  unsigned char* pkey;
 };
 
 __device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
@@ -231,7 +290,9 @@ __device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
   s.dsum = base + o.dsum;
   s.part = base + o.part;
   s.best = base + o.best;
-  s.pgeo = base + o.pgeo;
+  s.prec = base + o.prec;
+  s.perm = reinterpret_cast<int*>(base + o.perm);
+  s.pkey = reinterpret_cast<unsigned char*>(base + o.pkey);
   return s;
 }
 
@@ -271,10 +332,6 @@ __device__ __forceinline__ void stage_tables(const PsDevProblem& P, const Lds& L
   __syncthreads();
 }
 
-// Basis value of shape column q (parameter q+2) at facet f.
-__device__ __forceinline__ double ybasis(const PsDevProblem& P, const double* ytab, int f, int q) {
-  return ytab[((size_t)f * 16 + (q & 15)) * P.nbs + (q >> 4)];
-}
 
 // Facet areas of parameter vector a (Start.cu:421-528) into nrm[4f+3].
 __device__ __forceinline__ void curv_block(const PsDevProblem& P, const double* a, double* nrm) {
@@ -304,42 +361,43 @@ struct PointScalars {
   double sf, cf, tt, scale, ff, d2, alph;
 };
 
-__device__ __forceinline__ PointScalars point_scalars(const PsDevProblem& P, const Inv& v, int pt) {
+__device__ __forceinline__ PointScalars point_scalars(const PsDevProblem& P, const Inv& v, double tt, double alph) {
   PointScalars q;
-  q.tt = P.tim[pt];
+  q.tt = tt;
   double f = fma(v.omega, q.tt, P.phi0);
   f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
   psm::sincos(f, &q.sf, &q.cf);
-  q.alph = P.alph[pt];
+  q.alph = alph;
   q.ff = psm::exp2(-1.44269504088896 * (q.alph * v.invD));
   q.scale = fma(v.kph, q.alph, fma(v.A, q.ff, 1.0));
   q.d2 = ((v.A * q.ff) * q.alph) * v.invD2;
   return q;
 }
 
-// Phase 1 for one 16-point tile of one curve, executed by one wave.
+// Phase 1 for facet-pass tile t, the points perm[16t .. 16t+15] of the
+// evaluation's view-direction order, executed by one wave.  A point's raw
+// Jacobian row depends on that point alone, so the grouping changes no
+// result; points that see the same facets share the facet steps with any
+// visible facet, and the steps without one are skipped.  The rows go to the
+// slab at each point's own index.
 template <int NB, int NBS>
-__device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, const double* pgeo, int p0, int lp,
-                                                int t, double* jbuf, double* ybuf, double* tsum, double* rowbuf) {
+__device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds& L, const double* prec, const int* perm,
+                                                int t, double* jbuf, double* ybuf, double* rowbuf) {
   const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
-  const int F = P.nfac, QS = P.qs, C = P.ncoef;
-  const int pin = t * 16 + ps;
-  const bool valid = pin < lp;
-  const int pt = p0 + (valid ? pin : 0);
+  const int F = P.nfac, QS = P.qs, C = P.ncoef, N = P.ndata;
+  const int pos = t * 16 + ps;
+  const bool valid = pos < N;
+  const int pt = valid ? perm[pos] : 0;
   (void)NB;
-  // point geometry (the evaluation's per-point table): only the body-frame
-  // directions stay live in the facet loop; padding lanes see nothing
-  const int N = P.ndata;
+  // body-frame directions from the point's record (the only point values
+  // live in the facet loop); padding lanes see nothing
   double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0};
   const double cl = L.inv[5], cls = L.inv[6];
   if (valid) {
-    const double* o = pgeo + pt;
-    g[0] = o[5 * N];
-    g[1] = o[6 * N];
-    g[2] = o[7 * N];
-    g0[0] = o[8 * N];
-    g0[1] = o[9 * N];
-    g0[2] = o[10 * N];
+    const double* o = prec + (size_t)pt * PS_REC;
+    const double sf = o[RC_SF], cf = o[RC_CF];
+    body_dir(cf, sf, o[RC_VX], o[RC_UY], o[RC_VZ], g);
+    body_dir(cf, sf, o[RC_VX0], o[RC_UY0], o[RC_VZ0], g0);
   }
   double br = 0.0, t4 = 0.0, t5 = 0.0;
   double U0 = 0.0, U1 = 0.0, U2 = 0.0, V0 = 0.0, V1 = 0.0, V2 = 0.0;
@@ -433,22 +491,24 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
     // loads are not merged with the ones before the facet loop
     const int z = lane >> 6;
     const Inv v = load_inv(L.inv + z);
-    const int pq = pt + z;
     double sf = 0.0, cf = 0.0, tt = 0.0, scale = 0.0, ff = 0.0, d2 = 0.0, alph = 0.0;
+    Frame fe{0.0, 0.0, 0.0, 0.0}, fs{0.0, 0.0, 0.0, 0.0};
     if (valid) {
-      const double* o = pgeo + pq;
-      sf = o[0];
-      cf = o[N];
-      scale = o[2 * N];
-      ff = o[3 * N];
-      d2 = o[4 * N];
-      tt = P.tim[pq];
-      alph = P.alph[pq];
+      const double* o = prec + (size_t)(pt + z) * PS_REC;
+      sf = o[RC_SF];
+      cf = o[RC_CF];
+      fe = Frame{o[RC_UX], o[RC_UY], o[RC_VX], o[RC_VZ]};
+      fs = Frame{o[RC_UX0], o[RC_UY0], o[RC_VX0], o[RC_VZ0]};
+      tt = o[RC_TT];
+      scale = o[RC_SCALE];
+      ff = o[RC_FF];
+      d2 = o[RC_D2];
+      alph = o[RC_ALPH];
     }
-    body_vectors(v, cf, sf, tt, valid ? P.ex[pq] : 0.0, valid ? P.ey[pq] : 0.0, valid ? P.ez[pq] : 0.0, gg, db,
-                 dl, dw);
-    body_vectors(v, cf, sf, tt, valid ? P.sx[pq] : 0.0, valid ? P.sy[pq] : 0.0, valid ? P.sz[pq] : 0.0, gg,
-                 db0, dl0, dw0);
+    body_dir(cf, sf, fe.vx, fe.uy, fe.vz, gg);
+    body_derivs(v, cf, sf, tt, fe, gg, db, dl, dw);
+    body_dir(cf, sf, fs.vx, fs.uy, fs.vz, gg);
+    body_derivs(v, cf, sf, tt, fs, gg, db0, dl0, dw0);
     auto tsum = [&](const double* dv, const double* dv0) {
       double r = dv[0] * U0;
       r = fma(dv[1], U1, r);
@@ -473,24 +533,19 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
     rb[9] = sc;
   }
   wave_sync();
-  // rows out (global point index p0 + prow) and the tile's column sums:
-  // lane group sl sums the points 4r + sl in order, the groups combine as
-  // (G0 + G1) + (G2 + G3); padding rows add +0.0
-  double tsq[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) tsq[b] = 0.0;
-  double tsy = 0.0;
-  // Blocks below NBS - 1 hold shape columns only (NBS = ceil((C - 1) / 16));
-  // FULL: all 16 points of the tile exist (every tile but a curve's last)
+  // rows out: lane (ps, sl) writes column 16b + ps of the tile's points
+  // 4r + sl.  Blocks below NBS - 1 hold shape columns only (NBS =
+  // ceil((C - 1) / 16)); FULL: all 16 points of the tile exist (every tile
+  // but the last)
   auto rows_out = [&](auto full_tag) __attribute__((always_inline)) {
     constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int ptl = 4 * r + sl;
-      const int prow = t * 16 + ptl;
-      const bool ok = FULL || prow < lp;
+      const bool ok = FULL || t * 16 + ptl < N;
+      const int prow = __shfl(pt, ptl);  // lane ptl holds the point of tile row ptl
       const double* rb = rowbuf + ptl * 16;
-      double* jr = jbuf + (size_t)(p0 + (ok ? prow : 0)) * QS;
+      double* jr = jbuf + (size_t)prow * QS;
       const double sc9 = rb[9];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -504,27 +559,84 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
           val = (b < NBS && q <= C - 2) ? sc9 * gat[b < NBS ? b : 0][r] : ns;
         }
         if (FULL || ok) jr[q] = val;
-        tsq[b] = tsq[b] + (FULL ? val : (ok ? val : 0.0));
       }
-      const double yv = rb[8];
-      if (ok && ps == 0) ybuf[p0 + prow] = yv;
-      tsy = tsy + (FULL ? yv : (ok ? yv : 0.0));
+      if (ok && ps == 0) ybuf[prow] = rb[8];
     }
   };
-  if (t * 16 + 16 <= lp)
+  if (t * 16 + 16 <= N)
     rows_out(std::true_type{});
   else
     rows_out(std::false_type{});
+  wave_sync();
+  return nvis;
+}
+
+// Column sums of the data tiles (16 consecutive points of one curve,
+// aligned at the curve start) from the rows in the slab: lane (ps, sl) sums
+// column 16b + ps of the points 4r + sl in order, the lane groups combine as
+// (G0 + G1) + (G2 + G3); padding rows add +0.0.  Column qs: brightness.
+template <int NB>
+struct TileRows {
+  double v[4][NB], y[4];
+  int n;  // points in the tile
+};
+
+template <int NB>
+__device__ __forceinline__ void tile_rows_load(const PsDevProblem& P, const Slab& S, int T, TileRows<NB>& tr) {
+  const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
+  const int QS = P.qs;
+  const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
+  tr.n = ti.y;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ptl = 4 * r + sl;
+    const int prow = ti.x + (ptl < ti.y ? ptl : 0);
+    const double* jr = S.jbuf + (size_t)prow * QS;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) tr.v[r][b] = jr[16 * b + ps];
+    tr.y[r] = S.ybuf[prow];
+  }
+}
+
+template <int NB>
+__device__ __forceinline__ void tile_rows_sum(const PsDevProblem& P, const Slab& S, int T, const TileRows<NB>& tr) {
+  const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
+  const int QS = P.qs;
+  double tsq[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) tsq[b] = 0.0;
+  double tsy = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const bool ok = 4 * r + sl < tr.n;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) tsq[b] = tsq[b] + (ok ? tr.v[r][b] : 0.0);
+    tsy = tsy + (ok ? tr.y[r] : 0.0);
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b) tsq[b] = xsum16_32(tsq[b]);
   tsy = xsum16_32(tsy);
+  double* ts = S.tsum + (size_t)T * (QS + 1);
   if (sl == 0) {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) tsum[16 * b + ps] = tsq[b];
-    if (ps == 0) tsum[QS] = tsy;
+    for (int b = 0; b < NB; ++b) ts[16 * b + ps] = tsq[b];
+    if (ps == 0) ts[QS] = tsy;
   }
-  wave_sync();
-  return nvis;
+}
+
+// Wave w's data tiles T = w (mod NW), two at a time: every load of both
+// tiles is issued before the first sum.
+template <int NB>
+__device__ __forceinline__ void tile_sums(const PsDevProblem& P, const Slab& S, int w) {
+  for (int T = w; T < P.ntiles; T += 2 * NW) {
+    const int T2 = T + NW;
+    const bool two = T2 < P.ntiles;
+    TileRows<NB> a, b;
+    tile_rows_load<NB>(P, S, T, a);
+    if (two) tile_rows_load<NB>(P, S, T2, b);
+    tile_rows_sum<NB>(P, S, T, a);
+    if (two) tile_rows_sum<NB>(P, S, T2, b);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -707,30 +819,52 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, in a scalar register
   const int mf = P.mfit, K = mf + 1, QS = P.qs, C = P.ncoef, F = P.nfac, N = P.ndata;
   PS_TSTART();
+  // view-direction cell counts, in the normal-matrix area (dead until the
+  // end of the evaluation; the facet pass later reuses it for its rows)
+  int* cell = reinterpret_cast<int*>(L.work);
+  for (int e = tid; e < kCells; e += WG) cell[e] = 0;
   curv_block(P, a, L.nrm);
-  if (tid == 0) store_inv(make_inv(P, a), L.inv);
+  int* next_tile = reinterpret_cast<int*>(L.red);  // phase-1 work counter (red is free until the solve)
+  if (tid == 0) {
+    store_inv(make_inv(P, a), L.inv);
+    *next_tile = 0;
+  }
   __syncthreads();
-  // per-point rotation and phase-function terms and the body-frame
-  // directions of Earth and Sun (Start.cu:2095-2195), once per evaluation
+  // per-point rotation and phase-function terms and the pole-frame
+  // directions of Earth and Sun (Start.cu:2095-2195), once per evaluation,
+  // and each point's view-direction cell
   {
     const Inv v = load_inv(L.inv);
     for (int pt = tid; pt < N; pt += WG) {
-      const PointScalars q = point_scalars(P, v, pt);
-      double g[3], g0[3], scr[3];
-      body_vectors(v, q.cf, q.sf, q.tt, P.ex[pt], P.ey[pt], P.ez[pt], g, scr, scr, scr);
-      body_vectors(v, q.cf, q.sf, q.tt, P.sx[pt], P.sy[pt], P.sz[pt], g0, scr, scr, scr);
-      double* o = S.pgeo + pt;
-      o[0] = q.sf;
-      o[N] = q.cf;
-      o[2 * N] = q.scale;
-      o[3 * N] = q.ff;
-      o[4 * N] = q.d2;
-      o[5 * N] = g[0];
-      o[6 * N] = g[1];
-      o[7 * N] = g[2];
-      o[8 * N] = g0[0];
-      o[9 * N] = g0[1];
-      o[10 * N] = g0[2];
+      // the point's inputs, loaded together ahead of the arithmetic
+      const double tt = P.tim[pt], al = P.alph[pt];
+      const double ex = P.ex[pt], ey = P.ey[pt], ez = P.ez[pt];
+      const double sx = P.sx[pt], sy = P.sy[pt], sz = P.sz[pt];
+      const PointScalars q = point_scalars(P, v, tt, al);
+      const Frame fe = pole_frame(v, ex, ey, ez);
+      const Frame fs = pole_frame(v, sx, sy, sz);
+      double* o = S.prec + (size_t)pt * PS_REC;
+      o[RC_SF] = q.sf;
+      o[RC_CF] = q.cf;
+      o[RC_VX] = fe.vx;
+      o[RC_UY] = fe.uy;
+      o[RC_VZ] = fe.vz;
+      o[RC_UX] = fe.ux;
+      o[RC_VX0] = fs.vx;
+      o[RC_UY0] = fs.uy;
+      o[RC_VZ0] = fs.vz;
+      o[RC_UX0] = fs.ux;
+      o[RC_TT] = q.tt;
+      o[RC_SCALE] = q.scale;
+      o[RC_FF] = q.ff;
+      o[RC_D2] = q.d2;
+      o[RC_ALPH] = q.alph;
+      double g[3], g0[3];
+      body_dir(q.cf, q.sf, fe.vx, fe.uy, fe.vz, g);
+      body_dir(q.cf, q.sf, fs.vx, fs.uy, fs.vz, g0);
+      const int key = view_cell(g, g0);
+      S.pkey[pt] = (unsigned char)key;
+      atomicAdd(cell + key, 1);
     }
   }
   PS_TMARK(0);
@@ -739,26 +873,68 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   for (int idx = tid; idx < 3 * QS; idx += WG) {
     const int j = idx / QS, q = idx - j * QS;
     double s = 0.0;
-    if (q <= C - 2)
-      for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3] * L.nrm[4 * f + j], ybasis(P, L.ytab, f, q), s);
+    if (q <= C - 2) {
+      const double* yq = L.ytab + (size_t)(q & 15) * P.nbs + (q >> 4);
+      const size_t ys_ = (size_t)16 * P.nbs;  // basis stride of one facet
+#pragma unroll 8
+      for (int f = 0; f < F; ++f) s = fma(L.nrm[4 * f + 3] * L.nrm[4 * f + j], yq[f * ys_], s);
+    }
     S.jbuf[(size_t)(N + j) * QS + q] = s;
     if (q == QS - 1) {
       double ys = 0.0;
+#pragma unroll 8
       for (int f = 0; f < F; ++f) ys = ys + L.nrm[4 * f + 3] * L.nrm[4 * f + j];
       S.ybuf[N + j] = ys;
     }
   }
-  __syncthreads();  // the per-point table is complete
-  // phase 1 over every tile of every curve
+  __syncthreads();  // the point records and the cell counts are complete
+  // counting sort of the points by cell: exclusive prefix of the counts
+  // (wave 0; lane l owns cells 4l .. 4l + 3), then each point takes the
+  // next slot of its cell.  The order within a cell depends on timing; no
+  // result depends on the order.
+  if (w == 0) {
+    int c4[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = 4 * lane + j;
+      c4[j] = e < kCells ? cell[e] : 0;
+      sum += c4[j];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    int base = incl - sum;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = 4 * lane + j;
+      if (e < kCells) cell[e] = base;
+      base += c4[j];
+    }
+  }
+  __syncthreads();
+  for (int pt = tid; pt < N; pt += WG) S.perm[atomicAdd(cell + S.pkey[pt], 1)] = pt;
+  __syncthreads();  // the facet-pass order is complete
+  PS_TMARK(9);
+  // phase 1 over the points in view-direction order, 16 per tile; each wave
+  // takes the next unclaimed tile (tiles differ in cost by their visible
+  // facet steps; the assignment changes no result)
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
-  for (int T = w; T < P.ntiles; T += NW) {
-    // the tile's first point and its number of points (one load)
-    const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
-    nvis += phase1_tile<NB, NBS>(P, L, S.pgeo, ti.x, ti.y, 0, S.jbuf, S.ybuf, S.tsum + (size_t)T * (QS + 1),
-                                 rowbuf);
+  const int n_tiles = (N + 15) >> 4;
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(next_tile, 1);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= n_tiles) break;
+    nvis += phase1_tile<NB, NBS>(P, L, S.prec, S.perm, t, S.jbuf, S.ybuf, rowbuf);
   }
   PS_TMARK(1);
+  __syncthreads();
+  // column sums of the data tiles, in file order
+  tile_sums<NB>(P, S, w);
   __syncthreads();
   PS_TMARK(2);
   // curve sums: the tile sums of each curve in order (column qs: brightness,

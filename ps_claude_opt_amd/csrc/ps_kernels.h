@@ -72,6 +72,9 @@ struct PsFitArgs {
 // groups) and the groups' partial matrices are added at the end.
 __host__ __device__ inline int ps_tile_chains(int nb) { return nb <= 4 ? 4 : 2; }
 
+// Doubles per point in the per-evaluation point record (one 128-byte line).
+#define PS_REC 16
+
 // One workgroup's scratch slab in HBM (offsets in doubles, 256-byte aligned).
 struct PsSlab {
   size_t jbuf;   // [ndata+3][qs] raw Jacobian rows (data points, then the 3 convexity rows)
@@ -80,8 +83,10 @@ struct PsSlab {
   size_t dsum;   // [ncurves][qs+1] per-curve column sums (column qs: 1 / brightness sum)
   size_t part;   // [chains][16nb][16nb] partial augmented normal matrices
   size_t best;   // [K][K] accepted normal matrix
-  size_t pgeo;   // [11][ndata] per-point terms of the evaluated parameters: sin and cos of the
-                 // rotation angle, scale, ff, d2, body-frame Earth (3) and Sun (3) directions
+  size_t prec;   // [ndata][PS_REC] per-point record of the evaluated parameters (rotation angle,
+                 // pole-frame Earth and Sun directions, phase-function terms)
+  size_t perm;   // [ndata] ints: the points in the order of the facet-pass tiles
+  size_t pkey;   // [ndata] bytes: each point's view-direction cell
   size_t stride;
 };
 
@@ -103,8 +108,12 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
   o += al((size_t)ps_tile_chains(P.nb) * 256 * P.nb * P.nb);
   s.best = o;
   o += al(K * K);
-  s.pgeo = o;
-  o += al((size_t)11 * P.ndata);
+  s.prec = o;
+  o += al((size_t)PS_REC * P.ndata);
+  s.perm = o;
+  o += al(((size_t)P.ndata + 1) / 2);
+  s.pkey = o;
+  o += al(((size_t)P.ndata + 7) / 8);
   s.stride = o;
   return s;
 }
