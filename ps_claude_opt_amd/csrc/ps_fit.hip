@@ -361,12 +361,16 @@ struct PointScalars {
   double sf, cf, tt, scale, ff, d2, alph;
 };
 
+__device__ __forceinline__ void rotation_angle(const PsDevProblem& P, const Inv& v, double tt, double* sf, double* cf) {
+  double f = fma(v.omega, tt, P.phi0);
+  f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
+  psm::sincos(f, sf, cf);
+}
+
 __device__ __forceinline__ PointScalars point_scalars(const PsDevProblem& P, const Inv& v, double tt, double alph) {
   PointScalars q;
   q.tt = tt;
-  double f = fma(v.omega, q.tt, P.phi0);
-  f = fma(-(2.0 * kPi), round(f * (1.0 / (2.0 * kPi))), f);
-  psm::sincos(f, &q.sf, &q.cf);
+  rotation_angle(P, v, tt, &q.sf, &q.cf);
   q.alph = alph;
   q.ff = psm::exp2(-1.44269504088896 * (q.alph * v.invD));
   q.scale = fma(v.kph, q.alph, fma(v.A, q.ff, 1.0));
@@ -389,12 +393,13 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   const bool valid = pos < N;
   const int pt = valid ? perm[pos] : 0;
   (void)NB;
-  // body-frame directions from the point's record (the only point values
-  // live in the facet loop); padding lanes see nothing
+  // body-frame directions from the point's record (records are stored in
+  // the sorted order; the directions are the only point values live in the
+  // facet loop); padding lanes see nothing
   double g[3] = {0.0, 0.0, 0.0}, g0[3] = {0.0, 0.0, 0.0};
   const double cl = L.inv[5], cls = L.inv[6];
   if (valid) {
-    const double* o = prec + (size_t)pt * PS_REC;
+    const double* o = prec + (size_t)pos * PS_REC;
     const double sf = o[RC_SF], cf = o[RC_CF];
     body_dir(cf, sf, o[RC_VX], o[RC_UY], o[RC_VZ], g);
     body_dir(cf, sf, o[RC_VX0], o[RC_UY0], o[RC_VZ0], g0);
@@ -415,15 +420,16 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   const double* __restrict__ nr = L.nrm + 4 * sl;
   const double* __restrict__ ytab = L.ytab;  // uniform base; step k's rows at ytab + 64 NBS k
   const int ly = (16 * sl + ps) * NBS;       // this lane's offset in a step's rows
-  unsigned nvis = 0;  // this lane's visible (point, facet) pairs
+  unsigned nvis = 0;  // the wave's visible (point, facet) pairs (wave-uniform)
   auto step = [&](double n0, double n1, double n2, double ar, const double* y) {
     const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
     const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
     const bool vis = (mu > kTiny) && (mu0 > kTiny);
-    if (__builtin_amdgcn_ballot_w64(vis) != 0) {  // a whole step of invisible facets adds exact zeros
+    const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vis);
+    if (vmask != 0) {  // a whole step of invisible facets adds exact zeros
+      nvis += (unsigned)__builtin_popcountll(vmask);
       double w = 0.0;
       if (vis) {
-        ++nvis;
         const double rd = recip(mu + mu0);
         const double mm = mu * mu0;
         const double sc = mm * fma(cls, rd, cl);
@@ -494,7 +500,7 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
     double sf = 0.0, cf = 0.0, tt = 0.0, scale = 0.0, ff = 0.0, d2 = 0.0, alph = 0.0;
     Frame fe{0.0, 0.0, 0.0, 0.0}, fs{0.0, 0.0, 0.0, 0.0};
     if (valid) {
-      const double* o = prec + (size_t)(pt + z) * PS_REC;
+      const double* o = prec + (size_t)(pos + z) * PS_REC;
       sf = o[RC_SF];
       cf = o[RC_CF];
       fe = Frame{o[RC_UX], o[RC_UY], o[RC_VX], o[RC_VZ]};
@@ -568,7 +574,7 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   else
     rows_out(std::false_type{});
   wave_sync();
-  return nvis;
+  return lane == 0 ? nvis : 0u;
 }
 
 // Column sums of the data tiles (16 consecutive points of one curve,
@@ -836,32 +842,16 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   {
     const Inv v = load_inv(L.inv);
     for (int pt = tid; pt < N; pt += WG) {
-      // the point's inputs, loaded together ahead of the arithmetic
-      const double tt = P.tim[pt], al = P.alph[pt];
+      const double tt = P.tim[pt];
       const double ex = P.ex[pt], ey = P.ey[pt], ez = P.ez[pt];
       const double sx = P.sx[pt], sy = P.sy[pt], sz = P.sz[pt];
-      const PointScalars q = point_scalars(P, v, tt, al);
+      double sf, cf;
+      rotation_angle(P, v, tt, &sf, &cf);
       const Frame fe = pole_frame(v, ex, ey, ez);
       const Frame fs = pole_frame(v, sx, sy, sz);
-      double* o = S.prec + (size_t)pt * PS_REC;
-      o[RC_SF] = q.sf;
-      o[RC_CF] = q.cf;
-      o[RC_VX] = fe.vx;
-      o[RC_UY] = fe.uy;
-      o[RC_VZ] = fe.vz;
-      o[RC_UX] = fe.ux;
-      o[RC_VX0] = fs.vx;
-      o[RC_UY0] = fs.uy;
-      o[RC_VZ0] = fs.vz;
-      o[RC_UX0] = fs.ux;
-      o[RC_TT] = q.tt;
-      o[RC_SCALE] = q.scale;
-      o[RC_FF] = q.ff;
-      o[RC_D2] = q.d2;
-      o[RC_ALPH] = q.alph;
       double g[3], g0[3];
-      body_dir(q.cf, q.sf, fe.vx, fe.uy, fe.vz, g);
-      body_dir(q.cf, q.sf, fs.vx, fs.uy, fs.vz, g0);
+      body_dir(cf, sf, fe.vx, fe.uy, fe.vz, g);
+      body_dir(cf, sf, fs.vx, fs.uy, fs.vz, g0);
       const int key = view_cell(g, g0);
       S.pkey[pt] = (unsigned char)key;
       atomicAdd(cell + key, 1);
@@ -915,8 +905,39 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
     }
   }
   __syncthreads();
-  for (int pt = tid; pt < N; pt += WG) S.perm[atomicAdd(cell + S.pkey[pt], 1)] = pt;
-  __syncthreads();  // the facet-pass order is complete
+  // each point's record, computed again (the same arithmetic, so the same
+  // values) and stored at its place in the sorted order
+  {
+    const Inv v = load_inv(L.inv);
+    for (int pt = tid; pt < N; pt += WG) {
+      // the point's inputs, loaded together ahead of the arithmetic
+      const double tt = P.tim[pt], al = P.alph[pt];
+      const double ex = P.ex[pt], ey = P.ey[pt], ez = P.ez[pt];
+      const double sx = P.sx[pt], sy = P.sy[pt], sz = P.sz[pt];
+      const int pos = atomicAdd(cell + S.pkey[pt], 1);
+      const PointScalars q = point_scalars(P, v, tt, al);
+      const Frame fe = pole_frame(v, ex, ey, ez);
+      const Frame fs = pole_frame(v, sx, sy, sz);
+      double* o = S.prec + (size_t)pos * PS_REC;
+      o[RC_SF] = q.sf;
+      o[RC_CF] = q.cf;
+      o[RC_VX] = fe.vx;
+      o[RC_UY] = fe.uy;
+      o[RC_VZ] = fe.vz;
+      o[RC_UX] = fe.ux;
+      o[RC_VX0] = fs.vx;
+      o[RC_UY0] = fs.uy;
+      o[RC_VZ0] = fs.vz;
+      o[RC_UX0] = fs.ux;
+      o[RC_TT] = q.tt;
+      o[RC_SCALE] = q.scale;
+      o[RC_FF] = q.ff;
+      o[RC_D2] = q.d2;
+      o[RC_ALPH] = q.alph;
+      S.perm[pos] = pt;
+    }
+  }
+  __syncthreads();  // the facet-pass order and the records are complete
   PS_TMARK(9);
   // phase 1 over the points in view-direction order, 16 per tile; each wave
   // takes the next unclaimed tile (tiles differ in cost by their visible

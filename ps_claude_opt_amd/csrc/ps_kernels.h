@@ -83,8 +83,8 @@ struct PsSlab {
   size_t dsum;   // [ncurves][qs+1] per-curve column sums (column qs: 1 / brightness sum)
   size_t part;   // [chains][16nb][16nb] partial augmented normal matrices
   size_t best;   // [K][K] accepted normal matrix
-  size_t prec;   // [ndata][PS_REC] per-point record of the evaluated parameters (rotation angle,
-                 // pole-frame Earth and Sun directions, phase-function terms)
+  size_t prec;   // [ndata][PS_REC] per-point records of the evaluated parameters (rotation angle,
+                 // pole-frame Earth and Sun directions, phase-function terms), in facet-pass order
   size_t perm;   // [ndata] ints: the points in the order of the facet-pass tiles
   size_t pkey;   // [ndata] bytes: each point's view-direction cell
   size_t stride;
