@@ -588,10 +588,9 @@ struct TileRows {
 };
 
 template <int NB>
-__device__ __forceinline__ void tile_rows_load(const PsDevProblem& P, const Slab& S, int T, TileRows<NB>& tr) {
+__device__ __forceinline__ void tile_rows_load(const PsDevProblem& P, const Slab& S, int4 ti, TileRows<NB>& tr) {
   const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
   const int QS = P.qs;
-  const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
   tr.n = ti.y;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -630,18 +629,29 @@ __device__ __forceinline__ void tile_rows_sum(const PsDevProblem& P, const Slab&
   }
 }
 
-// Wave w's data tiles T = w (mod NW), two at a time: every load of both
-// tiles is issued before the first sum.
+// Wave w's data tiles T = w (mod NW), TT at a time: every row load of a
+// group is issued before its first sum, and the next group's tile
+// descriptors are loaded while the rows are in flight.
 template <int NB>
 __device__ __forceinline__ void tile_sums(const PsDevProblem& P, const Slab& S, int w) {
-  for (int T = w; T < P.ntiles; T += 2 * NW) {
-    const int T2 = T + NW;
-    const bool two = T2 < P.ntiles;
-    TileRows<NB> a, b;
-    tile_rows_load<NB>(P, S, T, a);
-    if (two) tile_rows_load<NB>(P, S, T2, b);
-    tile_rows_sum<NB>(P, S, T, a);
-    if (two) tile_rows_sum<NB>(P, S, T2, b);
+  constexpr int TT = NB <= 4 ? 3 : 2;  // tiles in flight per wave
+  const int nt = P.ntiles;
+  auto info = [&](int T) __attribute__((always_inline)) {
+    return T < nt ? *reinterpret_cast<const int4*>(P.tile_info + 4 * T) : make_int4(0, 0, 0, 0);
+  };
+  int4 ti[TT];
+#pragma unroll
+  for (int k = 0; k < TT; ++k) ti[k] = info(w + k * NW);
+  for (int T = w; T < nt; T += TT * NW) {
+    TileRows<NB> tr[TT];
+#pragma unroll
+    for (int k = 0; k < TT; ++k)
+      if (T + k * NW < nt) tile_rows_load<NB>(P, S, ti[k], tr[k]);
+#pragma unroll
+    for (int k = 0; k < TT; ++k) ti[k] = info(T + (TT + k) * NW);
+#pragma unroll
+    for (int k = 0; k < TT; ++k)
+      if (T + k * NW < nt) tile_rows_sum<NB>(P, S, T + k * NW, tr[k]);
   }
 }
 
