@@ -846,9 +846,8 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
     *next_tile = 0;
   }
   __syncthreads();
-  // per-point rotation and phase-function terms and the pole-frame
-  // directions of Earth and Sun (Start.cu:2095-2195), once per evaluation,
-  // and each point's view-direction cell
+  // each point's view-direction cell, from its rotation angle and the
+  // body-frame directions of Earth and Sun (Start.cu:2095-2195)
   {
     const Inv v = load_inv(L.inv);
     for (int pt = tid; pt < N; pt += WG) {
@@ -887,11 +886,11 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
       S.ybuf[N + j] = ys;
     }
   }
-  __syncthreads();  // the point records and the cell counts are complete
+  __syncthreads();  // the cells and their counts are complete
   // counting sort of the points by cell: exclusive prefix of the counts
-  // (wave 0; lane l owns cells 4l .. 4l + 3), then each point takes the
-  // next slot of its cell.  The order within a cell depends on timing; no
-  // result depends on the order.
+  // (wave 0; lane l owns cells 4l .. 4l + 3, kCells <= 256), then each point
+  // takes the next slot of its cell.  The order within a cell depends on
+  // timing; no result depends on the order.
   if (w == 0) {
     int c4[4], sum = 0;
 #pragma unroll
@@ -915,8 +914,10 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
     }
   }
   __syncthreads();
-  // each point's record, computed again (the same arithmetic, so the same
-  // values) and stored at its place in the sorted order
+  // each point's record (Start.cu:2095-2195: rotation angle, pole-frame
+  // Earth and Sun directions, phase-function terms), computed with the same
+  // arithmetic as the cells above and stored at the point's place in the
+  // sorted order
   {
     const Inv v = load_inv(L.inv);
     for (int pt = tid; pt < N; pt += WG) {
