@@ -725,37 +725,46 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
   // last block then needs the per-lane column kind.
   auto tiles = [&](auto allj_tag) __attribute__((always_inline)) {
     constexpr bool ALLJ = decltype(allj_tag)::value;
+    // first point, points in the tile, curve, curve length (one load);
+    // past the last tile: a one-point tile at point 0
+    auto info = [&](int T) __attribute__((always_inline)) {
+      return T < P.ntiles ? *reinterpret_cast<const int4*>(P.tile_info + 4 * T) : make_int4(0, 1, 0, 1);
+    };
+    auto load = [&](int4 t, int r) __attribute__((always_inline)) {
+      Row R;
+      const int pin = 4 * r + sl;
+      const int pt = t.x + (pin < t.y ? pin : 0);
+      const double* jr = S.jbuf + (size_t)pt * QS + ps;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b];
+      R.y = S.ybuf[pt];
+      R.sig = P.sig[pt];
+      R.obs = P.bright[pt];
+      R.s2 = P.s2w[pt];
+      return R;
+    };
+    int4 ti = info(gt);
+    Row nxt = load(ti, 0);
     for (int T = gt; T < P.ntiles; T += SH::GT) {
-      // first point, points in the tile, curve, curve length (one load)
-      const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
-      const int tp0 = ti.x, n_in = ti.y, c = ti.z, lp = ti.w;
+      const int4 tn = info(T + SH::GT);
+      const int n_in = ti.y, c = ti.z, lp = ti.w;
       const double* ds = S.dsum + (size_t)c * (QS + 1);
       double dv[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) dv[b] = ds[16 * b + ps];
       const double rave = ds[QS];
       const double lpd = (double)lp;
-      auto load = [&](int r) __attribute__((always_inline)) {
-        Row R;
-        const int pin = 4 * r + sl;
-        const int pt = tp0 + (pin < n_in ? pin : 0);
-        const double* jr = S.jbuf + (size_t)pt * QS + ps;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b];
-        R.y = S.ybuf[pt];
-        R.sig = P.sig[pt];
-        R.obs = P.bright[pt];
-        R.s2 = P.s2w[pt];
-        return R;
-      };
       // FULL: the tile has 16 points (every tile but a curve's last one)
       auto rows = [&](auto full_tag) __attribute__((always_inline)) {
         constexpr bool FULL = decltype(full_tag)::value;
         Row rw[2];
-        rw[0] = load(0);
+        rw[0] = nxt;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          if (r < 3) rw[(r + 1) & 1] = load(r + 1);
+          if (r < 3)
+            rw[(r + 1) & 1] = load(ti, r + 1);
+          else
+            nxt = load(tn, 0);
           const Row& cur = rw[r & 1];
           const bool ok = FULL || 4 * r + sl < n_in;
           // relative renormalisation (Start.cu:2656-2715)
@@ -783,6 +792,7 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
         rows(std::true_type{});
       else
         rows(std::false_type{});
+      ti = tn;
     }
   };
   if (16 * (NB - 1) <= mf)
@@ -1008,14 +1018,54 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   PS_TMARK(6);
 }
 
+// Pivot candidates are ordered by |entry|, then by position (row * 128 +
+// column): the larger position wins a tie, which is the NR rule's last
+// occurrence in row-major order.  |entry| >= 0 is compared through its bit
+// pattern (monotonic for non-negative doubles; NaN never becomes a
+// candidate); "no candidate" is (0, -1), below every real candidate.
+__device__ __forceinline__ bool cand_better(unsigned long long ok, int oi, unsigned long long k, int i) {
+  return ok > k || (ok == k && oi > i);
+}
+
+// One step of the wave argmax: combine with the candidate that DPP control
+// CTRL brings in (lanes outside ROWMASK keep their own).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void cand_step(unsigned long long& k, int& i) {
+  const int lo = (int)(unsigned)k, hi = (int)(unsigned)(k >> 32);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xF, false);
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, ROWMASK, 0xF, false);
+  const unsigned long long ok = ((unsigned long long)(unsigned)ohi << 32) | (unsigned)olo;
+  if (cand_better(ok, oi, k, i)) {
+    k = ok;
+    i = oi;
+  }
+}
+
+// The wave's best candidate, in lane 63: quad butterflies, the half-row and
+// row mirrors (every lane of a 16-lane row then holds the row's best), then
+// the row broadcasts 15 (rows 1, 3) and 31 (rows 2, 3).
+__device__ __forceinline__ void wave_argmax(unsigned long long& k, int& i) {
+  cand_step<0xB1, 0xF>(k, i);   // quad_perm [1, 0, 3, 2]
+  cand_step<0x4E, 0xF>(k, i);   // quad_perm [2, 3, 0, 1]
+  cand_step<0x141, 0xF>(k, i);  // row_half_mirror
+  cand_step<0x140, 0xF>(k, i);  // row_mirror
+  cand_step<0x142, 0xA>(k, i);  // row_bcast:15
+  cand_step<0x143, 0xC>(k, i);  // row_bcast:31
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)k, 63);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
+  i = __builtin_amdgcn_readlane(i, 63);
+  k = ((unsigned long long)hi << 32) | lo;
+}
+
 // Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) by the whole
 // workgroup on the augmented matrix [A | b] in LDS (rows 0..mf-1, stride
 // K = mf + 1, b in column mf).  Lane l of every wave owns columns l and
 // l + 64; wave w owns the rows ll = w (mod NW).  NR gaussj pivot rule: the
 // largest |entry| among unpivoted rows and columns, the last occurrence in
 // row-major order on ties; each wave searches its rows while eliminating
-// them and the waves' candidates meet in red[0 .. 2*NW).  Two barriers per
-// pivot.  Returns 0, or 2 for a zero pivot (or nothing selectable); the
+// them, reduces its lanes' candidates with DPP, and the waves' candidates
+// meet in red[0 .. 2*NW).  Two barriers per pivot.  Returns 0, or 2 for a zero pivot (or nothing selectable); the
 // solution is left in column mf.  Every thread of the workgroup must call it.
 // TWO: columns beyond 64 exist (K > 64).
 template <bool TWO>
@@ -1040,32 +1090,32 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
     if (c1 < mf) consider(a[ll * K + c1], ll * 128 + c1);
   }
   for (int i = 0; i < mf; ++i) {
-    // block argmax of (|value|, row, column)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(big, off);
-      const int oi = __shfl_xor(bidx, off);
-      if (oi >= 0 && (bidx < 0 || ob > big || (ob == big && oi > bidx))) {
-        big = ob;
-        bidx = oi;
-      }
-    }
+    unsigned long long key = bidx >= 0 ? (unsigned long long)__double_as_longlong(big) : 0ull;
+    int kidx = bidx;
+    wave_argmax(key, kidx);
+    unsigned long long* ured = reinterpret_cast<unsigned long long*>(red);
+    long long* ired = reinterpret_cast<long long*>(red + NW);
     if (lane == 0) {
-      red[w] = big;
-      red[NW + w] = (double)bidx;
+      ured[w] = key;
+      ired[w] = kidx;
     }
     __syncthreads();
-    big = 0.0;
-    bidx = -1;
+    unsigned long long wk[NW];
+    int wi[NW];
 #pragma unroll
     for (int ww = 0; ww < NW; ++ww) {
-      const double ob = red[ww];
-      const int oi = (int)red[NW + ww];
-      if (oi >= 0 && (bidx < 0 || ob > big || (ob == big && oi > bidx))) {
-        big = ob;
-        bidx = oi;
-      }
+      wk[ww] = ured[ww];
+      wi[ww] = (int)ired[ww];
     }
+    key = 0;
+    kidx = -1;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww)
+      if (cand_better(wk[ww], wi[ww], key, kidx)) {
+        key = wk[ww];
+        kidx = wi[ww];
+      }
+    bidx = __builtin_amdgcn_readfirstlane(kidx);
     if (bidx < 0) return 2;
     const int irow = bidx >> 7, icol = bidx & 127;
     if (icol < 64)
