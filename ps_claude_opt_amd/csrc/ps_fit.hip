@@ -1065,7 +1065,10 @@ __device__ __forceinline__ void wave_argmax(unsigned long long& k, int& i) {
 // largest |entry| among unpivoted rows and columns, the last occurrence in
 // row-major order on ties; each wave searches its rows while eliminating
 // them, reduces its lanes' candidates with DPP, and the waves' candidates
-// meet in red[0 .. 2*NW).  Two barriers per pivot.  Returns 0, or 2 for a zero pivot (or nothing selectable); the
+// meet in red[0 .. 2*NW).  Two barriers per pivot: every wave reads rows
+// irow and icol before the second; after it, the waves that own those rows
+// store the swapped rows (icol: the scaled pivot row, irow: old row icol)
+// and every wave eliminates its rows, a whole wave's rows per batch of loads.  Returns 0, or 2 for a zero pivot (or nothing selectable); the
 // solution is left in column mf.  Every thread of the workgroup must call it.
 // TWO: columns beyond 64 exist (K > 64).
 template <bool TWO>
@@ -1135,8 +1138,7 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       pv1 = a[irow * K + c1];
       cv1 = a[icol * K + c1];
     }
-    const double piv = __shfl(icol < 64 ? pv0 : pv1, icol & 63);  // row irow, column icol
-    const double cpv = __shfl(icol < 64 ? cv0 : cv1, icol & 63);  // row icol, column icol
+    const double piv = a[irow * K + icol];  // row irow, column icol
     __syncthreads();
     if (piv == 0.0) {
       // the rows are swapped before the zero pivot is found
@@ -1155,17 +1157,23 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
     const double pivinv = 1.0 / piv;
     const double pn0 = (c0 == icol ? 1.0 : pv0) * pivinv;
     const double pn1 = (c1 == icol ? 1.0 : pv1) * pivinv;
+    // the row swap: row icol becomes the scaled pivot row, row irow old
+    // row icol (each written by the wave that owns it)
     if (icol % NW == w) {
       if (h0) a[icol * K + c0] = pn0;
       if (h1) a[icol * K + c1] = pn1;
     }
+    if (irow != icol && irow % NW == w) {
+      if (h0) a[irow * K + c0] = cv0;
+      if (h1) a[irow * K + c1] = cv1;
+    }
     big = 0.0;
     bidx = -1;
-    // eliminate this wave's rows; after the swap row irow holds old row icol
-    for (int l0 = w; l0 < mf; l0 += 4 * NW) {
-      double dm[4], b0[4], b1[4];
+    // eliminate this wave's rows
+    for (int l0 = w; l0 < mf; l0 += 16 * NW) {
+      double dm[16], b0[16], b1[16];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int ll = l0 + u * NW;
         const int lr = ll < mf ? ll : w;
         dm[u] = a[lr * K + icol];
@@ -1173,20 +1181,19 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
         b1[u] = h1 ? a[lr * K + c1] : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int ll = l0 + u * NW;
         if (ll >= mf || ll == icol) continue;
-        const bool sw = ll == irow;
-        const double dum = sw ? cpv : dm[u];
+        const double dum = dm[u];
         const bool cand = !used(ll);
         if (h0) {
-          const double base = (c0 == icol) ? 0.0 : (sw ? cv0 : b0[u]);
+          const double base = (c0 == icol) ? 0.0 : b0[u];
           const double nv = fma(-pn0, dum, base);
           a[ll * K + c0] = nv;
           if (cand && open0) consider(nv, ll * 128 + c0);
         }
         if (h1) {
-          const double base = (c1 == icol) ? 0.0 : (sw ? cv1 : b1[u]);
+          const double base = (c1 == icol) ? 0.0 : b1[u];
           const double nv = fma(-pn1, dum, base);
           a[ll * K + c1] = nv;
           if (cand && open1) consider(nv, ll * 128 + c1);
