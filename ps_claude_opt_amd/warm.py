@@ -180,15 +180,22 @@ def fit_standardisation(x: np.ndarray, y: np.ndarray):
     return xm.astype(np.float32), xs, ym.astype(np.float32), ys
 
 
-def make_training_set(scan, n_pairs_per_problem: int, conw_r: Sequence[float], seed: int = 0):
+def make_training_set(scan, n_pairs_per_problem: int, conw_r: Sequence[float], seed: int = 0,
+                      near_n: Optional[Sequence[int]] = None, near_frac: float = 0.0, near_width: int = 2):
     """Cold fits of seeded random (frequency, pole) pairs of every problem of
     the scan's batch; returns (features, targets = final - initial
     parameters, results, problem index) of the fits that converged to finite
-    parameters."""
+    parameters.  With near_n (each problem's true frequency index), a
+    fraction near_frac of a problem's pairs is drawn within near_width
+    frequency steps of it: the pairs whose converged shape is the
+    asteroid's, the ones a truncated scan must rank first."""
     rng = np.random.default_rng(seed)
     prob, nl, ml = [], [], []
     for i, b in enumerate(scan.batch_info):
         n = rng.integers(1, b["n_max"] + 1, n_pairs_per_problem)
+        if near_n is not None and near_frac > 0:
+            k = int(round(near_frac * n_pairs_per_problem))
+            n[:k] = np.clip(near_n[i] + rng.integers(-near_width, near_width + 1, k), 1, b["n_max"])
         prob += [i] * n_pairs_per_problem
         nl += list(n)
         ml += list(rng.integers(1, 11, n_pairs_per_problem))

@@ -29,8 +29,8 @@ def _write_one(args):
     path, index = args
     from ps_claude_opt_amd import synth
 
-    synth.write_asteroid(path, "c2", synth.config_seed(4, index))
-    return path
+    truth = synth.write_asteroid(path, "c2", synth.config_seed(4, index))
+    return path, truth["period_h"]
 
 
 def _one_thread():
@@ -50,6 +50,13 @@ def generate(directory: str, first: int, count: int, workers: int):
         return pool.map(_write_one, jobs)
 
 
+def true_freq_index(scan, periods_h) -> np.ndarray:
+    """Each problem's scan frequency nearest to its true rotation frequency
+    (the scan's periods are 1 / (f_start - (n - 1) f_step) days)."""
+    return np.array([int(round((b["f_start"] - 24.0 / p) / b["f_step"])) + 1
+                     for b, p in zip(scan.batch_info, periods_h)])
+
+
 def initial_conw(scan) -> np.ndarray:
     return np.array([b["conw"] / b["escl"] / b["escl"] for b in scan.batch_info])
 
@@ -62,6 +69,9 @@ def main(argv=None) -> int:
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--near-frac", type=float, default=0.0,
+                    help="fraction of each asteroid's pairs drawn near its true frequency")
+    ap.add_argument("--near-width", type=int, default=2, help="frequency steps either side of the true one")
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--work-dir", default="gpurun_out/warm")
     ap.add_argument("--out", default="gpurun_out/warm/warm_lmax6.npz")
@@ -71,15 +81,18 @@ def main(argv=None) -> int:
     from ps_claude_opt_amd import warm
 
     t0 = time.time()
-    train_paths = generate(os.path.join(args.work_dir, "train"), 0, args.asteroids, args.workers)
-    val_paths = generate(os.path.join(args.work_dir, "val"), 100000, args.validation, args.workers)
+    train = generate(os.path.join(args.work_dir, "train"), 0, args.asteroids, args.workers)
+    val = generate(os.path.join(args.work_dir, "val"), 100000, args.validation, args.workers)
+    train_paths, val_paths = [p for p, _ in train], [p for p, _ in val]
     print(f"# generated {len(train_paths)} + {len(val_paths)} asteroids in {time.time() - t0:.1f} s", flush=True)
     report = dict(asteroids=args.asteroids, pairs_per_asteroid=args.pairs, epochs=args.epochs, batch=args.batch,
-                  lr=args.lr)
+                  lr=args.lr, near_frac=args.near_frac, near_width=args.near_width)
     with ps.Scan(0) as s:
         t0 = time.time()
         s.prepare_files(train_paths)
-        feats, tg, res, prob = warm.make_training_set(s, args.pairs, initial_conw(s), seed=11)
+        near = true_freq_index(s, [t for _, t in train])
+        feats, tg, res, prob = warm.make_training_set(s, args.pairs, initial_conw(s), seed=11, near_n=near,
+                                                      near_frac=args.near_frac, near_width=args.near_width)
         report.update(training_pairs=int(len(feats)), fit_s=round(time.time() - t0, 2))
         print(f"# {len(feats)} training pairs in {time.time() - t0:.1f} s", flush=True)
         t0 = time.time()
@@ -92,29 +105,37 @@ def main(argv=None) -> int:
         print(f"# trained in {time.time() - t0:.1f} s: loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         ws.save(args.out)
-        # validation on held-out asteroids
+        # validation on held-out asteroids: random pairs, and pairs within
+        # near_width frequency steps of the true frequency
         s.prepare_files(val_paths)
         rng = np.random.default_rng(99)
         k = 200
         pr = np.repeat(np.arange(len(val_paths)), k)
-        nl = np.concatenate([rng.integers(1, b["n_max"] + 1, k) for b in s.batch_info])
         ml = rng.integers(1, 11, pr.size)
         cw = initial_conw(s)[pr]
-        full, _ = s.fit_items_from(pr, nl, ml, cw, cw, None)
+        vnear = true_freq_index(s, [t for _, t in val])
+        sets = {"random": np.concatenate([rng.integers(1, b["n_max"] + 1, k) for b in s.batch_info]),
+                "near": np.concatenate([np.clip(vnear[i] + rng.integers(-args.near_width, args.near_width + 1, k), 1,
+                                                b["n_max"]) for i, b in enumerate(s.batch_info)])}
         net = ws.network(s)
-        a0 = ws.initial_params(s, net, pr, nl, ml)
-        net.close()
         val = {}
-        for it in (5, 10, 15, 20, 30):
-            c, _ = s.fit_items_from(pr, nl, ml, cw, cw, None, n_iter=it)
-            w, _ = s.fit_items_from(pr, nl, ml, cw, cw, a0, n_iter=it)
-            ok = np.isfinite(full["dev"]) & (full["dev"] < 1e30)
-            gc = np.abs(c["dev"] - full["dev"])[ok]
-            gw = np.abs(w["dev"] - full["dev"])[ok]
-            val[str(it)] = dict(cold_median_gap=float(np.median(gc)), warm_median_gap=float(np.median(gw)),
-                                cold_within_5e4=float(np.mean(gc <= 5e-4)), warm_within_5e4=float(np.mean(gw <= 5e-4)))
-            print(f"# {it} iterations: |dev - dev_full| median cold {np.median(gc):.2e} warm {np.median(gw):.2e}; "
-                  f"within 5e-4: cold {np.mean(gc <= 5e-4):.2f} warm {np.mean(gw <= 5e-4):.2f}", flush=True)
+        for name, nl in sets.items():
+            full, _ = s.fit_items_from(pr, nl, ml, cw, cw, None)
+            a0 = ws.initial_params(s, net, pr, nl, ml)
+            val[name] = {}
+            for it in (5, 10, 15, 20, 30):
+                c, _ = s.fit_items_from(pr, nl, ml, cw, cw, None, n_iter=it)
+                w, _ = s.fit_items_from(pr, nl, ml, cw, cw, a0, n_iter=it)
+                ok = np.isfinite(full["dev"]) & (full["dev"] < 1e30)
+                gc = np.abs(c["dev"] - full["dev"])[ok]
+                gw = np.abs(w["dev"] - full["dev"])[ok]
+                val[name][str(it)] = dict(cold_median_gap=float(np.median(gc)), warm_median_gap=float(np.median(gw)),
+                                          cold_within_5e4=float(np.mean(gc <= 5e-4)),
+                                          warm_within_5e4=float(np.mean(gw <= 5e-4)))
+                print(f"# {name} pairs, {it} iterations: |dev - dev_full| median cold {np.median(gc):.2e} "
+                      f"warm {np.median(gw):.2e}; within 5e-4: cold {np.mean(gc <= 5e-4):.2f} "
+                      f"warm {np.mean(gw <= 5e-4):.2f}", flush=True)
+        net.close()
         report["validation"] = val
     with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:
         json.dump(report, f, indent=1)
