@@ -1068,8 +1068,9 @@ __device__ __forceinline__ void wave_argmax(unsigned long long& k, int& i) {
 // meet in red[0 .. 2*NW).  Two barriers per pivot: every wave reads rows
 // irow and icol before the second; after it, the waves that own those rows
 // store the swapped rows (icol: the scaled pivot row, irow: old row icol)
-// and every wave eliminates its rows, a whole wave's rows per batch of loads.  Returns 0, or 2 for a zero pivot (or nothing selectable); the
-// solution is left in column mf.  Every thread of the workgroup must call it.
+// and every wave eliminates its rows, loading up to 16 of them per batch.
+// Returns 0, or 2 for a zero pivot (or nothing selectable); the solution is
+// left in column mf.  Every thread of the workgroup must call it.
 // TWO: columns beyond 64 exist (K > 64).
 template <bool TWO>
 __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* red) {
