@@ -209,9 +209,17 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
     from ps_claude_opt_amd import warm
 
     ws = warm.WarmStart.load(WARM_WEIGHTS)
-    net = ws.network(scan)
     out = {"n_iter": n_iter, "refit_frequencies": refit, "weights": os.path.relpath(WARM_WEIGHTS, ROOT)}
-    for name, w in (("warm", ws), ("truncated_cold", None)):
+    modes = [("warm", ws), ("truncated_cold", None)]
+    if ws.y_mean.size != scan.info["ncoef"] - 1 + 4:
+        # the shipped network predicts the lmax-6 coefficients only: the
+        # truncated scan from the reference's start still runs beside the
+        # plain one
+        out["warm"] = {"skipped": f"the network predicts {ws.y_mean.size} parameters; this problem has "
+                                  f"{scan.info['ncoef'] - 1 + 4}"}
+        modes = modes[1:]
+    net = ws.network(scan) if len(modes) == 2 else None
+    for name, w in modes:
         t0 = time.perf_counter()
         res, refitted = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=refit)
         dt = time.perf_counter() - t0
@@ -224,7 +232,8 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
                      "best_line": b, "agrees_with_plain": bool(agree)}
         print(f"# {name}: {n_max * 10} pairs in {dt:.2f} s, best {b['period_h']:.6f} h rms {b['rms']:.6f} "
               f"agrees {agree}", file=sys.stderr, flush=True)
-    net.close()
+    if net is not None:
+        net.close()
     return out
 
 
