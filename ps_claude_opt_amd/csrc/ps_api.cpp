@@ -45,6 +45,28 @@ int hipfail(ps_ctx* c, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hipfail(ctx, e_, #call); \
   } while (0)
 
+// Device buffers of one call (the test entry points), freed on every
+// return path, error returns included.
+struct CallBufs {
+  std::vector<void*> p;
+  CallBufs() = default;
+  CallBufs(const CallBufs&) = delete;
+  CallBufs& operator=(const CallBufs&) = delete;
+  ~CallBufs() {
+    for (void* q : p) hipFree(q);
+  }
+  template <typename T>
+  hipError_t alloc(T** out, size_t n) {
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) {
+      p.push_back(q);
+      *out = static_cast<T*>(q);
+    }
+    return e;
+  }
+};
+
 template <typename T>
 int upload(ps_ctx* c, const T* host, size_t n, const T** dev) {
   void* p = nullptr;
@@ -1032,9 +1054,10 @@ int ps_eval(ps_ctx* c, const double* a, double conw_r2_rows, double* alpha, doub
   if (rc) return rc;
   const int K = c->mfit + 1, ma = c->ma, mf = c->mfit;
   double *da = nullptr, *dw = nullptr, *dar = nullptr, *dj = nullptr;
-  PS_HIP(c, hipMalloc(&da, ma * sizeof(double)));
-  PS_HIP(c, hipMalloc(&dw, (size_t)K * K * sizeof(double)));
-  PS_HIP(c, hipMalloc(&dar, c->nfac * sizeof(double)));
+  CallBufs bufs;
+  PS_HIP(c, bufs.alloc(&da, ma));
+  PS_HIP(c, bufs.alloc(&dw, (size_t)K * K));
+  PS_HIP(c, bufs.alloc(&dar, c->nfac));
   PS_HIP(c, hipMemcpy(da, a, ma * sizeof(double), hipMemcpyHostToDevice));
   PsFitArgs A = make_args(c, 1);
   A.conw_r2_rows = conw_r2_rows;
@@ -1054,15 +1077,11 @@ int ps_eval(ps_ctx* c, const double* a, double conw_r2_rows, double* alpha, doub
   }
   if (jrows) {
     const size_t nj = (size_t)(c->hp[0].ndata + 3) * (ma + 1);
-    PS_HIP(c, hipMalloc(&dj, nj * sizeof(double)));
+    PS_HIP(c, bufs.alloc(&dj, nj));
     PS_HIP(c, psw4::ps_launch_eval(A, c->stream, da, dw, dar, dj));
     PS_HIP(c, hipStreamSynchronize(c->stream));
     PS_HIP(c, hipMemcpy(jrows, dj, nj * sizeof(double), hipMemcpyDeviceToHost));
-    hipFree(dj);
   }
-  hipFree(da);
-  hipFree(dw);
-  hipFree(dar);
   return PS_OK;
 }
 
@@ -1072,20 +1091,17 @@ int ps_gaussj(ps_ctx* c, int mfit, const double* alpha, const double* beta, doub
   PS_HIP(c, hipSetDevice(c->device));
   double *da = nullptr, *db = nullptr, *dx = nullptr;
   int* de = nullptr;
-  PS_HIP(c, hipMalloc(&da, (size_t)mfit * mfit * sizeof(double)));
-  PS_HIP(c, hipMalloc(&db, mfit * sizeof(double)));
-  PS_HIP(c, hipMalloc(&dx, mfit * sizeof(double)));
-  PS_HIP(c, hipMalloc(&de, sizeof(int)));
+  CallBufs bufs;
+  PS_HIP(c, bufs.alloc(&da, (size_t)mfit * mfit));
+  PS_HIP(c, bufs.alloc(&db, mfit));
+  PS_HIP(c, bufs.alloc(&dx, mfit));
+  PS_HIP(c, bufs.alloc(&de, 1));
   PS_HIP(c, hipMemcpy(da, alpha, (size_t)mfit * mfit * sizeof(double), hipMemcpyHostToDevice));
   PS_HIP(c, hipMemcpy(db, beta, mfit * sizeof(double), hipMemcpyHostToDevice));
   PS_HIP(c, psw4::ps_launch_gaussj(mfit, da, db, lambda, dx, de, c->stream));
   PS_HIP(c, hipStreamSynchronize(c->stream));
   PS_HIP(c, hipMemcpy(x, dx, mfit * sizeof(double), hipMemcpyDeviceToHost));
   PS_HIP(c, hipMemcpy(err, de, sizeof(int), hipMemcpyDeviceToHost));
-  hipFree(da);
-  hipFree(db);
-  hipFree(dx);
-  hipFree(de);
   return PS_OK;
 }
 
@@ -1093,17 +1109,15 @@ int ps_math_probe(ps_ctx* c, int n, const double* x, double* out, double* out2) 
   if (!c || n <= 0 || !x || !out || !out2) return PS_E_ARG;
   PS_HIP(c, hipSetDevice(c->device));
   double *dx = nullptr, *d1 = nullptr, *d2 = nullptr;
-  PS_HIP(c, hipMalloc(&dx, n * sizeof(double)));
-  PS_HIP(c, hipMalloc(&d1, 3 * (size_t)n * sizeof(double)));
-  PS_HIP(c, hipMalloc(&d2, n * sizeof(double)));
+  CallBufs bufs;
+  PS_HIP(c, bufs.alloc(&dx, n));
+  PS_HIP(c, bufs.alloc(&d1, 3 * (size_t)n));
+  PS_HIP(c, bufs.alloc(&d2, n));
   PS_HIP(c, hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
   PS_HIP(c, psw4::ps_launch_math(n, dx, d1, d2, c->stream));
   PS_HIP(c, hipStreamSynchronize(c->stream));
   PS_HIP(c, hipMemcpy(out, d1, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   PS_HIP(c, hipMemcpy(out2, d2, n * sizeof(double), hipMemcpyDeviceToHost));
-  hipFree(dx);
-  hipFree(d1);
-  hipFree(d2);
   return PS_OK;
 }
 
