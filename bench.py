@@ -35,7 +35,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # HBM bytes per pair-fit of the fit kernel, from FETCH_SIZE / WRITE_SIZE
 # counter passes scaled by a plain copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ps_fit_kernel_traffic.json")
-WARM_WEIGHTS = os.path.join(ROOT, "ps_claude_opt_amd", "data", "warm_lmax6.npz")  # warm.DEFAULT_WEIGHTS
+WARM_DATA = os.path.join(ROOT, "ps_claude_opt_amd", "data")  # warm.weights_for(lmax): warm_lmax<lmax>.npz
 
 
 def measured_traffic(preset: str, pairs_per_launch: float):
@@ -185,8 +185,11 @@ def main() -> int:
                       "beta": best.beta_deg, "truth_period_h": truth["period_h"],
                       "truth_lambda": truth["lambda_deg"], "truth_beta": truth["beta_deg"]},
     }
-    if not args.no_warm and n_max == info["n_max"] and os.path.exists(WARM_WEIGHTS):
-        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters, args.warm_refit)
+    weights = os.path.join(WARM_DATA, f"warm_lmax{synth.PRESETS[args.preset].lmax}.npz")
+    if not os.path.exists(weights):
+        weights = os.path.join(WARM_DATA, "warm_lmax6.npz")
+    if not args.no_warm and n_max == info["n_max"] and os.path.exists(weights):
+        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters, args.warm_refit, weights)
     scan.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(path, conw_r, n_max, args.cpu_pairs)
@@ -194,7 +197,7 @@ def main() -> int:
     return 0
 
 
-def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: int) -> dict:
+def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: int, weights: str) -> dict:
     """The learned warm start on the same scan: every pair runs n_iter LM
     iterations from the network's prediction, then the `refit` best
     frequencies are refitted cold (the plain scan's own 50-iteration fits),
@@ -208,8 +211,8 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
 
     from ps_claude_opt_amd import warm
 
-    ws = warm.WarmStart.load(WARM_WEIGHTS)
-    out = {"n_iter": n_iter, "refit_frequencies": refit, "weights": os.path.relpath(WARM_WEIGHTS, ROOT)}
+    ws = warm.WarmStart.load(weights)
+    out = {"n_iter": n_iter, "refit_frequencies": refit, "weights": os.path.relpath(weights, ROOT)}
     modes = [("warm", ws), ("truncated_cold", None)]
     if ws.y_mean.size != scan.info["ncoef"] - 1 + 4:
         # the shipped network predicts the lmax-6 coefficients only: the

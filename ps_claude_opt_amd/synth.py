@@ -62,6 +62,12 @@ PRESETS: Dict[str, SynthConfig] = {
     "c5": SynthConfig(n_dense=4, pts_dense=200, n_sparse=20000, sparse_years=10.0,
                       sparse_group=250, lmax=8, nrows=8, n_freq=20000,
                       p_min_h=2.0, p_max_h=100.0),
+    # config 5's observing geometry (4 dense curves, sparse points over 10
+    # years, lmax 8) with 2,000 sparse points: the lmax-8 warm-start
+    # training asteroids
+    "c5s": SynthConfig(n_dense=4, pts_dense=200, n_sparse=2000, sparse_years=10.0,
+                       sparse_group=250, lmax=8, nrows=8, n_freq=20000,
+                       p_min_h=2.0, p_max_h=100.0),
     # small cases for tests
     "tiny": SynthConfig(n_dense=2, pts_dense=40, n_sparse=40, sparse_group=40,
                         lmax=3, nrows=3, n_freq=6),

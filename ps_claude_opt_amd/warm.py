@@ -34,6 +34,11 @@ N_LAYERS = 3
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "warm_lmax6.npz")
 
 
+def weights_for(lmax: int) -> str:
+    """The shipped weights for shapes of degree lmax (the file may be absent)."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", f"warm_lmax{lmax}.npz")
+
+
 class Mlp:
     """The network kernels through the C ABI (ps_mlp_*)."""
 

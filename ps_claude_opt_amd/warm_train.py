@@ -1,12 +1,16 @@
-"""Trains the lmax = 6 warm-start network on the GPU and reports what it
-gains on held-out asteroids.
+"""Trains a warm-start network on the GPU and reports what it gains on
+held-out asteroids.
 
-    python -m ps_claude_opt_amd.warm_train --asteroids 800 --pairs 40 \
+    python -m ps_claude_opt_amd.warm_train --asteroids 2000 --pairs 40 \
         --out ps_claude_opt_amd/data/warm_lmax6.npz
+    python -m ps_claude_opt_amd.warm_train --preset c5s --asteroids 2000 --pairs 40 \
+        --out ps_claude_opt_amd/data/warm_lmax8.npz
 
 Training data: cold LM fits (the input file's own 50-iteration stop mode) of
-seeded random (frequency, pole) pairs of config-2-style synthetic asteroids
-(seeds 20261203 + 4000 + i: disjoint from the benchmark's asteroid), each
+seeded random (frequency, pole) pairs of synthetic asteroids of the preset
+(c2: config-2-style, lmax 6; c5s: config 5's geometry with fewer sparse
+points, lmax 8; seeds 20261203 + 4000 + i: disjoint from the benchmark's
+asteroids), each
 fitted with its initial convexity weight conw / escl^2.  Features and targets
 as in warm.py; the network is trained by the library's kernels (BF16 / FP32
 accumulation, Adam).  Validation: on held-out asteroids, the rms reached by
@@ -26,10 +30,10 @@ import numpy as np
 
 
 def _write_one(args):
-    path, index = args
+    path, index, preset = args
     from ps_claude_opt_amd import synth
 
-    truth = synth.write_asteroid(path, "c2", synth.config_seed(4, index))
+    truth = synth.write_asteroid(path, preset, synth.config_seed(4, index))
     return path, truth["period_h"]
 
 
@@ -43,9 +47,9 @@ def _one_thread():
         pass
 
 
-def generate(directory: str, first: int, count: int, workers: int):
+def generate(directory: str, first: int, count: int, workers: int, preset: str = "c2"):
     os.makedirs(directory, exist_ok=True)
-    jobs = [(os.path.join(directory, f"a{first + i:05d}.in"), first + i) for i in range(count)]
+    jobs = [(os.path.join(directory, f"a{first + i:05d}.in"), first + i, preset) for i in range(count)]
     with Pool(workers, initializer=_one_thread) as pool:
         return pool.map(_write_one, jobs)
 
@@ -64,6 +68,7 @@ def initial_conw(scan) -> np.ndarray:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--asteroids", type=int, default=800)
+    ap.add_argument("--preset", default="c2", help="synthetic asteroid preset (c2: lmax 6; c5s: lmax 8)")
     ap.add_argument("--validation", type=int, default=16)
     ap.add_argument("--pairs", type=int, default=40, help="training pairs per asteroid")
     ap.add_argument("--epochs", type=int, default=60)
@@ -81,11 +86,11 @@ def main(argv=None) -> int:
     from ps_claude_opt_amd import warm
 
     t0 = time.time()
-    train = generate(os.path.join(args.work_dir, "train"), 0, args.asteroids, args.workers)
-    val = generate(os.path.join(args.work_dir, "val"), 100000, args.validation, args.workers)
+    train = generate(os.path.join(args.work_dir, "train"), 0, args.asteroids, args.workers, args.preset)
+    val = generate(os.path.join(args.work_dir, "val"), 100000, args.validation, args.workers, args.preset)
     train_paths, val_paths = [p for p, _ in train], [p for p, _ in val]
     print(f"# generated {len(train_paths)} + {len(val_paths)} asteroids in {time.time() - t0:.1f} s", flush=True)
-    report = dict(asteroids=args.asteroids, pairs_per_asteroid=args.pairs, epochs=args.epochs, batch=args.batch,
+    report = dict(preset=args.preset, asteroids=args.asteroids, pairs_per_asteroid=args.pairs, epochs=args.epochs, batch=args.batch,
                   lr=args.lr, near_frac=args.near_frac, near_width=args.near_width)
     with ps.Scan(0) as s:
         t0 = time.time()

@@ -251,3 +251,32 @@ def test_warm_scan_refits_are_plain_fits(inputs):
         assert w_best == p_best
     else:
         assert w_best["rms"] >= p_best["rms"]
+
+
+def test_lmax8_network_warm_fits_bit_identical_to_oracle(inputs):
+    """The shipped lmax-8 network (config 5's 84 predicted parameters) runs
+    on a config-5-shaped asteroid, and fits from its prediction equal the
+    oracle's bit for bit, final parameters included."""
+    import ps_claude_opt_amd as ps
+    from ps_claude_opt_amd import warm
+
+    path, _ = inputs("c5_small", "c5", None, n_sparse=2000)
+    O = po.Problem(path)
+    n = np.array([1, 700, 9000, 19999])
+    m = np.array([1, 5, 8, 10])
+    p = np.zeros(4, np.int32)
+    ws = warm.WarmStart.load(warm.weights_for(8))
+    with ps.Scan(0) as s:
+        s.prepare_file(path)
+        assert s.info["ncoef"] == 81 and ws.n_out == 84
+        net = ws.network(s)
+        a0 = ws.initial_params(s, net, p, n, m)
+        net.close()
+        cold = s.initial_params(p, n, m)
+        g, ga = s.fit_items_from(p, n, m, 0.8, 0.8, a0, want_final=True, n_iter=5)
+    assert np.isfinite(a0).all() and not np.array_equal(a0, cold)
+    o, oa = O.fit_pairs_from(n, m, 0.8, 0.8, a0, threads=8, want_final=True, n_iter=5)
+    for i in range(4):
+        for f in ("dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "niter", "n_vis"):
+            assert g[i][f] == float(getattr(o[i], f)) or (np.isnan(g[i][f]) and np.isnan(getattr(o[i], f))), (i, f)
+    assert np.array_equal(ga, oa, equal_nan=True)
