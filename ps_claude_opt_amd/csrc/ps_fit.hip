@@ -1293,7 +1293,12 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
           L.work[e] = vv;
         }
         __syncthreads();
+        // the solve is latency-bound: its waves issue first while the other
+        // workgroups on the CU run their facet passes (only the arbitration
+        // order changes)
+        __builtin_amdgcn_s_setprio(2);
         const int err = gaussj_block<(NB > 4)>(L.work, mf, K, L.red);
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (err == 0 || err == 2)
           for (int q = tid; q < mf; q += WG) L.atry[P.cmap[q]] = L.cgg[P.cmap[q]] + L.work[q * K + mf];
