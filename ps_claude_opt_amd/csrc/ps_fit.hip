@@ -848,6 +848,9 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   // view-direction cell counts, in the normal-matrix area (dead until the
   // end of the evaluation; the facet pass later reuses it for its rows)
   int* cell = reinterpret_cast<int*>(L.work);
+  // the point passes up to the facet pass wait on memory: like the solve,
+  // they run at wave priority 2 beside the other workgroups' facet passes
+  __builtin_amdgcn_s_setprio(2);
   for (int e = tid; e < kCells; e += WG) cell[e] = 0;
   curv_block(P, a, L.nrm);
   int* next_tile = reinterpret_cast<int*>(L.red);  // phase-1 work counter (red is free until the solve)
@@ -959,6 +962,7 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
     }
   }
   __syncthreads();  // the facet-pass order and the records are complete
+  __builtin_amdgcn_s_setprio(0);
   PS_TMARK(9);
   // phase 1 over the points in view-direction order, 16 per tile; each wave
   // takes the next unclaimed tile (tiles differ in cost by their visible
