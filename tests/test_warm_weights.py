@@ -23,3 +23,22 @@ def test_shipped_weights_shapes(lmax):
 def test_default_weights_are_the_lmax6_ones():
     assert warm.DEFAULT_WEIGHTS == warm.weights_for(6)
     assert synth.PRESETS["c2"].lmax == 6 and synth.PRESETS["c5s"].lmax == 8 == synth.PRESETS["c5"].lmax
+
+
+def test_true_frequency_index_inverts_the_scan_grid():
+    """warm_train's near-truth sampling: the scan's n-th period is
+    1 / (f_start - (n - 1) f_step) days, and the true period maps back to
+    its nearest n."""
+    from types import SimpleNamespace
+
+    from ps_claude_opt_amd.warm_train import true_freq_index
+
+    bi = [dict(f_start=12.0, f_step=0.002), dict(f_start=6.5, f_step=0.0005)]
+    n_true = [1234, 77]
+    periods_h = [24.0 / (b["f_start"] - (n - 1) * b["f_step"]) for b, n in zip(bi, n_true)]
+    got = true_freq_index(SimpleNamespace(batch_info=bi), periods_h)
+    assert list(got) == n_true
+    # a period between two grid points maps to the nearer one
+    b = bi[0]
+    f_mid = b["f_start"] - (1234 - 1 + 0.3) * b["f_step"]
+    assert true_freq_index(SimpleNamespace(batch_info=[b]), [24.0 / f_mid])[0] == 1234
