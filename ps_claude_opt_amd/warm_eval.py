@@ -2,7 +2,7 @@
 winner, and what does the network add over the same truncated scan from
 the reference's start?
 
-    python -m ps_claude_opt_amd.warm_eval --asteroids 8 --iters 10,15,20,25 \
+    python -m ps_claude_opt_amd.warm_eval --asteroids 16 --iters 15,20 \
         --out ps_claude_opt_amd/data/warm_eval_report.json
 
 For each seeded config-2-style asteroid (seeds 20261203 + 2000 + 300 + i,
@@ -52,8 +52,8 @@ def winner_ranks(scan, ws, net, conw_r: float, n_max: int, iters):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--asteroids", type=int, default=8)
-    ap.add_argument("--iters", default="10,20")
+    ap.add_argument("--asteroids", type=int, default=16)
+    ap.add_argument("--iters", default="15,20")
     ap.add_argument("--refit", default="10,50", help="refit counts to summarise")
     ap.add_argument("--work-dir", default="gpurun_out/warm_eval")
     ap.add_argument("--out", default="gpurun_out/warm_eval/warm_eval_report.json")
