@@ -1,7 +1,7 @@
 """Trains a warm-start network on the GPU and reports what it gains on
 held-out asteroids.
 
-    python -m ps_claude_opt_amd.warm_train --asteroids 2000 --pairs 40 \
+    python -m ps_claude_opt_amd.warm_train --asteroids 20000 --pairs 40 \
         --out ps_claude_opt_amd/data/warm_lmax6.npz
     python -m ps_claude_opt_amd.warm_train --preset c5s --asteroids 2000 --pairs 40 \
         --out ps_claude_opt_amd/data/warm_lmax8.npz
