@@ -994,6 +994,38 @@ int gaussj(int mf, std::vector<double>& cov, std::vector<double>& x) {
   return 0;
 }
 
+/* Iter2's rms (Start.cu:5129-5248): with the areas of the accepted
+ * parameters a, v = sum_f Area_f n_f, rchisq = Chisq - |v|^2 conw_r^2 (the
+ * convexity rows' share of Chisq), dev = sqrt(rchisq / ndata) over the data
+ * points only (the reference divides by its ndata - 3), chck = |v|. */
+void iter2_dev(const Problem& P, const double* a, double chisq, double conw_r_dev, double* area, double* dev,
+               double* chck) {
+  const int nf = P.nfac;
+  curv(P, a, area);
+  double vv[3];
+  for (int d = 0; d < 3; d++) {
+    double s = 0;
+    for (int f = 0; f < nf; f++) s = std::fma(area[f], P.nor[d][f], s);
+    vv[d] = s;
+  }
+  double aa = std::fma(vv[2], vv[2], std::fma(vv[1], vv[1], vv[0] * vv[0]));
+  const double conwr2 = conw_r_dev * conw_r_dev;
+  double rchisq = std::fma(-aa, conwr2, chisq);
+  *dev = std::sqrt(rchisq / (double)(P.ndata));
+  *chck = std::sqrt(aa);
+}
+
+/* FinishPole's dark-facet percentage (Start.cu:5056-5125):
+ * 100 |sum A n| / sum A, NaN -> 1. */
+double dark_percent(const Problem& P, const double* area, double chck) {
+  double tot = 0;
+  for (int f = 0; f < P.nfac; f++) tot = tot + area[f];
+  tot = 1.0 / tot;
+  double dd = (chck * 100.0) * tot;
+  if (std::isnan(dd)) dd = 1.0;
+  return dd;
+}
+
 /* ------------------------------------------------------------------ */
 /* LM fit of one pair: Start.cu:4346-5248 as one sequential state machine */
 /* ------------------------------------------------------------------ */
@@ -1003,7 +1035,7 @@ struct FitOut {
 
 void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r_dev, pso_order ord,
               pso_pair_result* res, double* traj, int traj_cap, const double* a0 = nullptr,
-              double* a_out = nullptr, int n_iter_fixed = 0) {
+              double* a_out = nullptr, int n_iter_fixed = 0, bool a0_full = false) {
   const int ma = P.ma, mf = P.mfit, C = P.ncoef, nf = P.nfac;
   std::vector<double> cgg(ma), atry(ma), alpha(mf * mf), beta(mf), covar(mf * mf), da(mf), area(nf);
   /* Start.cu:4346-4418 */
@@ -1015,7 +1047,10 @@ void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r
   for (int i = 1; i <= 3; i++) cgg[C + 2 + i] = P.par[i];
   cgg[C + 6] = P.lcl;
   cgg[C + 7] = 1;
-  if (a0) {
+  if (a0 && a0_full) {
+    /* caller-given start for every parameter (pole and rotation rate too) */
+    for (int i = 0; i < ma; i++) cgg[i] = a0[i];
+  } else if (a0) {
     /* caller-given start: shape and phase-function / Lambert parameters;
        pole, rotation rate and cls as above */
     for (int i = 0; i < C; i++) cgg[i] = a0[i];
@@ -1077,18 +1112,7 @@ void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r
     /* Iter2 (Start.cu:5129-5248) */
     if (Niter == 1 || Chisq < Ochisq) {
       Ochisq = Chisq;
-      curv(P, cgg.data(), area.data());
-      double vv[3];
-      for (int d = 0; d < 3; d++) {
-        double s = 0;
-        for (int f = 0; f < nf; f++) s = std::fma(area[f], P.nor[d][f], s);
-        vv[d] = s;
-      }
-      double aa = std::fma(vv[2], vv[2], std::fma(vv[1], vv[1], vv[0] * vv[0]));
-      const double conwr2 = conw_r_dev * conw_r_dev;
-      double rchisq = std::fma(-aa, conwr2, Chisq);
-      dev_new = std::sqrt(rchisq / (double)(P.ndata));
-      chck = std::sqrt(aa);
+      iter2_dev(P, cgg.data(), Chisq, conw_r_dev, area.data(), &dev_new, &chck);
       double diff = dev_old - dev_new;
       if (diff > 1e-10) { iter_diff = diff; dev_old = dev_new; }
     }
@@ -1107,13 +1131,8 @@ void fit_pair(const Problem& P, int n, int m, double conw_r2_rows, double conw_r
   r.n_evals = nevals;
   r.n_vis = nvis;
   if (!(dev_new >= 1e40)) {
-    double tot = 0;
-    for (int f = 0; f < nf; f++) tot = tot + area[f];
-    tot = 1.0 / tot;
-    double dd = (chck * 100.0) * tot;
-    if (std::isnan(dd)) dd = 1.0;
     r.dev = dev_new;
-    r.dark = dd;
+    r.dark = dark_percent(P, area.data(), chck);
     r.period = (2 * PI) / cgg[C + 2];
     double la = RAD2DEG * cgg[C + 1];
     r.lambda_deg = la + (la < 0 ? 360 : 0);
@@ -1134,7 +1153,7 @@ int best_pole(const pso_pair_result* r /* N_POLES */) {
 
 double fit_many(const Problem& P, int npairs, const int* nl, const int* ml, double conw_r, double conw_r_dev,
                 pso_order ord, int threads, pso_pair_result* res, const double* a0 = nullptr,
-                double* a_out = nullptr, int n_iter_fixed = 0) {
+                double* a_out = nullptr, int n_iter_fixed = 0, bool a0_full = false) {
   if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
   if (threads < 1) threads = 1;
   if (threads > npairs) threads = npairs > 0 ? npairs : 1;
@@ -1145,8 +1164,9 @@ double fit_many(const Problem& P, int npairs, const int* nl, const int* ml, doub
     for (;;) {
       int i = next.fetch_add(1);
       if (i >= npairs) break;
-      fit_pair(P, nl[i], ml[i], cr2, conw_r_dev, ord, &res[i], nullptr, 0, a0 ? a0 + (size_t)i * P.ma : nullptr,
-               a_out ? a_out + (size_t)i * P.ma : nullptr, n_iter_fixed);
+      fit_pair(P, nl ? nl[i] : 1, ml ? ml[i] : 1, cr2, conw_r_dev, ord, &res[i], nullptr, 0,
+               a0 ? a0 + (size_t)i * P.ma : nullptr, a_out ? a_out + (size_t)i * P.ma : nullptr, n_iter_fixed,
+               a0_full);
     }
   };
   std::vector<std::thread> th;
@@ -1155,6 +1175,22 @@ double fit_many(const Problem& P, int npairs, const int* nl, const int* ml, doub
   for (auto& t : th) t.join();
   auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count();
+}
+
+/* The decision that ends one round of the convexity-weight search
+ * (start_CUDA.cu:747-789): the mean over the n_periods test periods of the
+ * best pole's dark-facet percentage (a period with no reported pole adds
+ * nothing, the mean still divides by n_periods); below 1 accepts the
+ * weight, otherwise it doubles. */
+void conw_decide(const pso_pair_result* res, int n_periods, double* conw_r, int* new_conw) {
+  double sum_dark = 0;
+  for (int n = 0; n < n_periods; n++) {
+    int b = best_pole(&res[n * N_POLES]);
+    if (b >= 0) sum_dark = sum_dark + res[n * N_POLES + b].dark;
+  }
+  double ave_dark = sum_dark / n_periods;
+  if (ave_dark < 1.0) *new_conw = 1;
+  if (ave_dark >= 1.0) *conw_r = *conw_r * 2;
 }
 
 /* start_CUDA.cu:400-789 + period_search_BOINC.cpp:647-771 */
@@ -1170,15 +1206,8 @@ int conw_search(Problem& P, int threads, pso_order ord, double* conw_out) {
     /* device conw_r during precalc is the bit pattern of a host pointer
        (start_CUDA.cu:476): a subnormal whose square is exactly 0 */
     fit_many(P, (int)nl.size(), nl.data(), ml.data(), conwR, 0.0, ord, threads, res.data());
-    double sum_dark = 0;
-    for (int n = 0; n < max_test_periods; n++) {
-      int b = best_pole(&res[n * N_POLES]);
-      if (b >= 0) sum_dark = sum_dark + res[n * N_POLES + b].dark;
-    }
-    double ave_dark = sum_dark / max_test_periods;
     rounds++;
-    if (ave_dark < 1.0) newConw = 1;
-    if (ave_dark >= 1.0) conwR = conwR * 2;
+    conw_decide(res.data(), max_test_periods, &conwR, &newConw);
   }
   *conw_out = conwR;
   return rounds;
@@ -1323,6 +1352,27 @@ double pso_fit_pairs_from(const pso_problem* h, int npairs, const int* n_list, c
 }
 
 void pso_set_stop(pso_problem* h, double stop_condition, int n_iter_min) { set_stop(h->P, stop_condition, n_iter_min); }
+
+double pso_fit_params(const pso_problem* h, int npairs, const double* a0, double conw_r, double conw_r_dev,
+                      int threads, int n_iter, pso_pair_result* res, double* a_out) {
+  return fit_many(h->P, npairs, nullptr, nullptr, conw_r, conw_r_dev, pso_order{4, 16}, threads, res, a0, a_out,
+                  n_iter, true);
+}
+
+void pso_finish(const pso_problem* h, const double* a, double chisq, double conw_r_dev, double* dev,
+                double* chck, double* dark) {
+  const Problem& P = h->P;
+  std::vector<double> area(P.nfac);
+  double d = 0, c = 0;
+  iter2_dev(P, a, chisq, conw_r_dev, area.data(), &d, &c);
+  if (dev) *dev = d;
+  if (chck) *chck = c;
+  if (dark) *dark = dark_percent(P, area.data(), c);
+}
+
+void pso_conw_decide(const pso_pair_result* res, int n_periods, double* conw_r, int* new_conw) {
+  conw_decide(res, n_periods, conw_r, new_conw);
+}
 
 }  // extern "C"
 

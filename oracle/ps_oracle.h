@@ -112,6 +112,22 @@ double pso_fit_pairs_from(const pso_problem* p, int npairs, const int* n_list, c
                           double conw_r, double conw_r_dev, int threads, const double* a0, int n_iter,
                           pso_pair_result* res, double* a_out);
 
+/* pso_fit_pairs_from with every parameter taken from a0[k*ma ...] (the
+ * pole and rotation rate too), so a fit can start at a known truth. */
+double pso_fit_params(const pso_problem* p, int npairs, const double* a0, double conw_r, double conw_r_dev,
+                      int threads, int n_iter, pso_pair_result* res, double* a_out);
+
+/* Iter2's rms and FinishPole's dark-facet percentage at parameters a with
+ * the given Chisq (the code path the fit itself uses): dev, chck = |sum A n|
+ * and dark (any pointer may be NULL). */
+void pso_finish(const pso_problem* p, const double* a, double chisq, double conw_r_dev, double* dev,
+                double* chck, double* dark);
+
+/* The convexity-weight decision of one search round on n_periods x 10 pair
+ * results: updates *conw_r (doubled unless accepted) and sets *new_conw = 1
+ * when the mean best-pole dark percentage is below 1. */
+void pso_conw_decide(const pso_pair_result* res, int n_periods, double* conw_r, int* new_conw);
+
 /* Test hook: override the LM iteration controls (n_iter_max, n_iter_min,
  * iter_diff_max) as if read from stopCondition / n_iter_min. */
 void pso_set_stop(pso_problem* p, double stop_condition, int n_iter_min);

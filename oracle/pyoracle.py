@@ -66,6 +66,11 @@ def load():
     L.pso_fit_pairs_from.restype = C.c_double
     L.pso_fit_pairs_from.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_double, C.c_double, C.c_int, _dp,
                                      C.c_int, C.POINTER(PairResult), _dp]
+    L.pso_fit_params.restype = C.c_double
+    L.pso_fit_params.argtypes = [C.c_void_p, C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.POINTER(PairResult), _dp]
+    L.pso_finish.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+    L.pso_conw_decide.argtypes = [C.POINTER(PairResult), C.c_int, _dp, _ip]
     _lib = L
     return L
 
@@ -162,6 +167,24 @@ class Problem:
                                   af.ctypes.data_as(_dp) if want_final else None)
         return list(out[:k]), af
 
+    def fit_params(self, a0, conw_r, conw_r_dev, threads=0, n_iter=0):
+        """Fits starting from full parameter vectors a0 [pairs][ma] (pole
+        and rotation rate included); returns (results, final params)."""
+        a0 = np.ascontiguousarray(np.atleast_2d(a0), np.float64)
+        k = a0.shape[0]
+        out = (PairResult * k)()
+        af = np.zeros((k, self.info["ma"]))
+        self.L.pso_fit_params(self.h, k, a0.ctypes.data_as(_dp), conw_r, conw_r_dev, threads, n_iter, out,
+                              af.ctypes.data_as(_dp))
+        return list(out), af
+
+    def finish(self, a, chisq, conw_r_dev):
+        """(dev, chck, dark) of Iter2 + FinishPole at parameters a."""
+        a = np.ascontiguousarray(a, np.float64)
+        d, c, k = C.c_double(), C.c_double(), C.c_double()
+        self.L.pso_finish(self.h, a.ctypes.data_as(_dp), chisq, conw_r_dev, C.byref(d), C.byref(c), C.byref(k))
+        return d.value, c.value, k.value
+
     def conw_search(self, threads=0, order=DEFAULT_ORDER) -> float:
         cw = C.c_double(0)
         self.L.pso_conw_search(self.h, threads, order, C.byref(cw))
@@ -185,6 +208,17 @@ def gaussj(alpha, beta, lam):
     x = np.zeros(mf)
     err = L.pso_gaussj(mf, alpha.ctypes.data_as(_dp), beta.ctypes.data_as(_dp), lam, x.ctypes.data_as(_dp))
     return x, err
+
+
+def conw_decide(results, conw_r):
+    """One conw round's decision on len(results) // 10 periods x 10 poles of
+    PairResult: returns (new conw_r, new_conw)."""
+    n = len(results) // 10
+    arr = (PairResult * (10 * n))(*results[:10 * n])
+    cw = C.c_double(conw_r)
+    nc = C.c_int(0)
+    load().pso_conw_decide(arr, n, C.byref(cw), C.byref(nc))
+    return cw.value, nc.value
 
 
 def exp(x):

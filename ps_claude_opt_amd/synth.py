@@ -152,6 +152,58 @@ class Truth(dict):
     pass
 
 
+def l1_columns(lmax: int):
+    """Indices of the three l = 1 columns of the Laplace basis in the
+    reference ordering (sphfunc.c:65-86): P_1^0 = z, P_1^1 cos(phi) = x,
+    P_1^1 sin(phi) = y (no Condon-Shortley phase)."""
+    return 1, lmax + 1, lmax + 2
+
+
+CLOSURE_TOL = 1e-12
+
+
+def close_gaussian_image(Y: np.ndarray, nrm: np.ndarray, coef: np.ndarray, lmax: int) -> np.ndarray:
+    """Sets the l = 1 coefficients so that the facet areas exp(Y @ coef)
+    satisfy sum_f A_f n_f = 0, the closure condition of a convex body's
+    Gaussian image (Minkowski): then a convex polyhedron with these normals
+    and areas exists.
+
+    The l = 1 terms of the series are exp(g . n): the closure is the zero
+    gradient of the strictly convex Phi(g) = sum_f exp(h_f + g . n_f), solved
+    by Newton's method."""
+    if lmax < 1:
+        raise ValueError("closing the Gaussian image needs lmax >= 1")
+    iz, ix, iy = l1_columns(lmax)
+    c = coef.copy()
+    c[[iz, ix, iy]] = 0.0
+    h = Y @ c
+    h -= h.max()
+    g = np.zeros(3)
+    w = np.exp(h)
+    for _ in range(100):
+        grad = nrm.T @ w
+        if np.linalg.norm(grad) <= 1e-15 * w.sum():
+            break
+        hess = (nrm * w[:, None]).T @ nrm
+        step = np.linalg.solve(hess, grad)
+        # damped Newton: halve the step until Phi decreases
+        phi = w.sum()
+        t = 1.0
+        while True:
+            w_new = np.exp(h + nrm @ (g - t * step))
+            if w_new.sum() <= phi or t < 1e-12:
+                break
+            t *= 0.5
+        g = g - t * step
+        w = w_new
+    c[ix], c[iy], c[iz] = g
+    area = np.exp(Y @ c)
+    gap = np.linalg.norm(nrm.T @ area) / area.sum()
+    if not gap < CLOSURE_TOL:
+        raise AssertionError(f"Gaussian image not closed: |sum A n| / sum A = {gap:.3e}")
+    return c
+
+
 def make_asteroid(cfg: SynthConfig, seed: int):
     """Returns (text of period_search_in, truth dict)."""
     rng = np.random.default_rng(seed)
@@ -162,9 +214,10 @@ def make_asteroid(cfg: SynthConfig, seed: int):
     # sigma 0.3/(l+1) per unit-rms basis function (the unnormalised columns
     # span several decades in magnitude)
     rms = np.sqrt(np.mean(Y * Y, axis=0))
-    coef = rng.normal(0.0, 1.0, Y.shape[1]) * (0.3 / (lv + 1.0)) / rms
-    coef[0] = 0.0
-    area = (4 * math.pi / nrm.shape[0]) * np.exp(Y @ coef)
+    shape_c = rng.normal(0.0, 1.0, Y.shape[1]) * (0.3 / (lv + 1.0)) / rms
+    shape_c[0] = 0.0
+    shape_c = close_gaussian_image(Y, nrm, shape_c, lmax)
+    area = (4 * math.pi / nrm.shape[0]) * np.exp(Y @ shape_c)
     period_h = rng.uniform(*cfg.period_range_h)
     lam = rng.uniform(0, 2 * math.pi)
     beta = math.asin(rng.uniform(-1, 1))
@@ -187,11 +240,18 @@ def make_asteroid(cfg: SynthConfig, seed: int):
         for g0 in range(0, cfg.n_sparse, cfg.sparse_group):
             sl_ = slice(g0, min(cfg.n_sparse, g0 + cfg.sparse_group))
             curves.append(dict(t=ts[sl_], sun=sun[sl_], earth=earth[sl_]))
+    # times as the file stores them (8 decimals); the rotation phase is
+    # referenced to the epoch the parser derives from them, (int) of the
+    # earliest time (period_search_BOINC.cpp:537-544), so the truth is a
+    # parameter vector of the fitted model itself
+    for cv in curves:
+        cv["t"] = np.round(cv["t"], 8)
+    jd0 = float(math.floor(min(float(cv["t"].min()) for cv in curves)))
     # render
     for cv in curves:
         t = cv["t"]
-        g = _rot_body(lam, beta, omega, phi0, t - jd_start, cv["earth"])
-        g0 = _rot_body(lam, beta, omega, phi0, t - jd_start, cv["sun"])
+        g = _rot_body(lam, beta, omega, phi0, t - jd0, cv["earth"])
+        g0 = _rot_body(lam, beta, omega, phi0, t - jd0, cv["sun"])
         mu = g @ nrm.T
         mu0 = g0 @ nrm.T
         vis = (mu > 0) & (mu0 > 0)
@@ -246,8 +306,28 @@ def make_asteroid(cfg: SynthConfig, seed: int):
     text = "\n".join(lines) + "\n"
     truth = dict(seed=seed, period_h=period_h, lambda_deg=math.degrees(lam) % 360.0,
                  beta_deg=math.degrees(beta), n_points=int(jd_all.size),
-                 n_curves=len(curves), n_freq=cfg.n_freq, lmax=lmax, nrows=cfg.nrows)
+                 n_curves=len(curves), n_freq=cfg.n_freq, lmax=lmax, nrows=cfg.nrows,
+                 jd0=jd0, shape=[float(x) for x in shape_c], par=list(cfg.par_true), cl=cfg.cl_true,
+                 noise=cfg.noise)
     return text, truth
+
+
+def truth_params(truth: dict) -> np.ndarray:
+    """The truth as a parameter vector of the fitted model (Appendix B
+    order: shape coefficients, beta' = 90 deg - beta, lambda, omega [rad/day],
+    phase-function A, D, k, log of the Lambert weight, Lommel-Seeliger
+    weight 1).  The rendering uses the parser's epoch and phi0 = 0, so this
+    vector reproduces the noise-free data up to the facet discretisation."""
+    shape = np.asarray(truth["shape"], float)
+    a = np.concatenate([shape, np.zeros(8)])
+    C = shape.size
+    a[C] = math.radians(90.0 - truth["beta_deg"])
+    a[C + 1] = math.radians(truth["lambda_deg"])
+    a[C + 2] = 2.0 * math.pi * 24.0 / truth["period_h"]
+    a[C + 3:C + 6] = truth["par"]
+    a[C + 6] = math.log(truth["cl"])
+    a[C + 7] = 1.0
+    return a
 
 
 def _header(cfg: SynthConfig, p_start, coef, p_end):
