@@ -17,7 +17,7 @@ flops of the fitted pairs / kernel time measured with HIP events on the
 library's stream) and a CPU baseline: the repository's CPU FP64 oracle on a
 seeded sample of pairs of the same workload, on the host's cores.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--cpu-pairs 128] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--cpu-pairs 512] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -49,12 +49,21 @@ def measured_traffic(preset: str, pairs_per_launch: float):
     return round((e["fetch_bytes_per_pair"] + e["write_bytes_per_pair"]) * pairs_per_launch)
 
 
-def _cores() -> int:
+def _affinity() -> int:
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        return os.cpu_count() or 1
+
+
+def _cores() -> int:
+    """The host cores this process has: its CPU affinity, or the share the
+    host allots it (OMP_NUM_THREADS) when that is smaller."""
+    n = _affinity()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
 
 
 def main() -> int:
@@ -63,7 +72,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--preset", default="c2")
-    ap.add_argument("--cpu-pairs", type=int, default=128)
+    ap.add_argument("--cpu-pairs", type=int, default=512)
     ap.add_argument("--asteroids", type=int, default=64, help="batched workload (--preset c3): asteroids")
     ap.add_argument("--max-freqs", type=int, default=0, help="scan only the first N frequencies (0 = all)")
     ap.add_argument("--conw-r", type=float, default=0.0, help="use this conwR instead of searching for it")
@@ -397,16 +406,20 @@ def cpu_baseline(path: str, conw_r: float, n_max: int, n_pairs: int) -> dict:
     nl = rng.integers(1, n_max + 1, n_pairs)
     ml = rng.integers(1, 11, n_pairs)
     cores = _cores()
+    print(f"# cpu baseline: {n_pairs} pairs on {cores} threads", file=sys.stderr, flush=True)
     _, wall = O.fit_pairs(nl, ml, conw_r, conw_r, threads=cores)
-    n1 = max(1, n_pairs // 32)
+    n1 = max(1, min(8, n_pairs))
     _, wall1 = O.fit_pairs(nl[:n1], ml[:n1], conw_r, conw_r, threads=1)
     return {
         "value": round(n_pairs / wall, 4),
         "unit": "pair-fits/s",
         "cores": cores,
+        "nproc": os.cpu_count(),
+        "affinity": _affinity(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         "kind": "port",
         "sample": f"{n_pairs} seeded random (frequency, pole) pairs of the same scan, {wall:.1f} s wall "
-                  f"on {cores} threads of the host CPU ({_cpu_model()}); the full scan would take "
+                  f"on {cores} threads (the host cores this process has, {_cpu_model()}); the full scan would take "
                   f"{n_max * 10 / (n_pairs / wall) / 3600:.2f} h",
         "single_core_value": round(n1 / wall1, 4),
         "single_core_sample": f"the first {n1} of those pairs on one thread, {wall1:.1f} s",
