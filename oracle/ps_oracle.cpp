@@ -40,15 +40,22 @@
  *     (aligned at the curve start; the last tile of a curve is padded);
  *     tiles are numbered over all curves in file order, the three
  *     convexity rows form one more tile after the last data tile;
- *   - column sums / curve brightness sums: per tile four partial sums over
- *     the tile's points p = s (mod 4) in order (padding adds +0.0),
- *     combined as (G0 + G1) + (G2 + G3); the curve sum adds the tile sums
- *     sequentially;
- *   - J^T W J, J^T W r and chi^2: one augmented product [J'|r]^T W [J'|r]
- *     split into G chains (G = 4 for rows of at most 64 columns, else 2):
- *     chain g holds the tiles T = g (mod G), every element a fused
- *     multiply-add chain over its tiles' points in order; the chains are
- *     combined as (C0 + C1) + (C2 + C3) (G = 4) or C0 + C1 (G = 2).
+ *   - curve brightness sums: per tile four partial sums over the tile's
+ *     points p = s (mod 4) in order (padding adds +0.0), combined as
+ *     (G0 + G1) + (G2 + G3); the curve sum adds the tile sums sequentially;
+ *   - J^T W J, J^T W r and chi^2: one augmented product (W Z)^T Z,
+ *     Z = [cJ|r|z|y], split into G chains (G = 4 for rows of at most 64
+ *     columns, else 2): chain g holds the tiles T = g (mod G), every element
+ *     a fused multiply-add chain (weighted side times plain side) over its
+ *     tiles' points in order; the chains are combined as
+ *     (C0 + C1) + (C2 + C3) (G = 4) or C0 + C1 (G = 2);
+ *   - per relative curve, the chain's rows z and y over its tiles of the
+ *     curve (the same chains, cleared after each curve), the chains
+ *     combined as the matrices (a chain without a tile of the curve gives
+ *     0.0);
+ *   - the renormalisation correction: one fused multiply-add chain per
+ *     element of the combined matrix over the curves in order, two terms
+ *     per curve (see mrqcof).
  * Transcendentals (exp, exp2, sin/cos of the LM parameters) use this file's
  * own Cody-Waite + Taylor implementations, which the kernel restates
  * identically; host-side setup uses the C library as the reference does.
@@ -783,7 +790,10 @@ long long point_row(const Problem& P, const PairInv& v, const double* area, int 
   double T1 = tsum(G.db, G.db0), T2 = tsum(G.dl, G.dl0), T3 = tsum(G.dw, G.dw0);
   const double sc = G.scale;
   row[0] = B * sc;
-  for (int c = 1; c <= C; c++) row[c] = sc * gat[c];
+  /* column 1, the size scale: every area carries the factor exp(a_1), so
+     d y / d a_1 = y */
+  row[1] = row[0];
+  for (int c = 2; c <= C; c++) row[c] = sc * gat[c];
   row[C + 1] = sc * T1;
   row[C + 2] = sc * T2;
   row[C + 3] = sc * T3;
@@ -805,7 +815,8 @@ void convexity_rows(const Problem& P, const double* area, double* rows /* 3*(ma+
     double ym = 0;
     for (int f = 0; f < nf; f++) ym = ym + area[f] * P.nor[j][f];
     row[0] = ym;
-    for (int c = 1; c <= C; c++) {
+    row[1] = ym; /* d y_j / d a_1 = y_j */
+    for (int c = 2; c <= C; c++) {
       double acc = 0;
       for (int f = 0; f < nf; f++) acc = std::fma(area[f] * P.nor[j][f], P.dsph[(c - 1) * nf + f], acc);
       row[c] = acc;
@@ -817,11 +828,11 @@ struct EvalOut {
   long long nvis = 0;
 };
 
-/* 16-column blocks of a Jacobian row as the device lays it out: the row
- * holds parameters 2..C+8 and the augmented normal matrix K = mfit + 1
- * columns. */
+/* 16-column blocks of a row as the device lays it out: the raw row holds
+ * parameters 2..C+8, the normal-equation rows mfit Jacobian columns, the
+ * residual and the two per-curve sum columns. */
 int row_blocks(const Problem& P) {
-  const int a = (P.ncoef + 7 + 15) / 16, b = (P.mfit + 1 + 15) / 16;
+  const int a = (P.ncoef + 7 + 15) / 16, b = (P.mfit + 3 + 15) / 16;
   return a > b ? a : b;
 }
 
@@ -831,15 +842,32 @@ int tile_chains(int nb) { return nb <= 4 ? 4 : 2; }
 
 /* One mrqcof evaluation (Start.cu:4464-4498, 2262-2828, 4791-4807):
  * alpha (mfit x mfit, symmetric), beta (mfit), chisq; area[] = curv(a).
- * Relative curves only on the GPU path; absolute data curves are handled by
- * the same math without renormalisation (the reference's GPU path does not
- * compute them, see Start.cu:4685-4716).
- * The convexity rows use sigr2 = conw_r2_rows and weight 1. */
+ * The convexity rows use sigr2 = conw_r2_rows and weight 1.
+ *
+ * Relative curves (Start.cu:2656-2715) are renormalised algebraically.  With
+ * L = sum y over the curve, c = sig lp / L and the renormalised row
+ * X = c (J - (y / L) d), d = sum J, the chains accumulate (W Z)^T Z over the
+ * rows Z = [c J | r | z | y] (r = obs - c y; z and y are the last two
+ * columns of the device's 16 NB-column rows; weighted side
+ * [w c J | w r | 1 | w y], the last two only on relative curves).  Row z
+ * then holds D = c d, row y holds U = c sum w y J, Y = sum w y^2 and
+ * R = sum w y r, written out and cleared at the end of each curve, and
+ *   X^T W X = [c J]^T W [c J] - (a b^T + b a^T),  a = U - (Y / 2L) D,
+ *   X^T W r = [c J]^T W r - R b,                  b = D / L,
+ * applied as one fused multiply-add chain per element over the curves.  So
+ * the rows are read once, with no per-curve pass before the product.
+ * Absolute curves (in_rel = 0) enter raw: X = J, r = obs - y; the column of
+ * parameter 1 (the size scale) is d y / d a_1 = y, zero for relative curves.
+ * The reference's GPU path computes no absolute data curves
+ * (Start.cu:4685-4716); this follows its legacy forms (Start.cu:3363-3692). */
 EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order ord,
                double* alpha, double* beta, double* chisq, double* area, double* jrows_out) {
   EvalOut out;
   const int ma = P.ma, mf = P.mfit, K = mf + 1; /* augmented width */
+  const int K2 = 16 * row_blocks(P);            /* the device's row width: sum columns z, y last */
+  const int ZC = K2 - 2, YC = K2 - 1;
   const int TL = 16; /* tile of 16 points: fixes the chain of every point */
+  const int NC = P.l_curves;
   (void)ord;
   curv(P, a, area);
   PairInv v = pair_invariants(P, a);
@@ -848,96 +876,159 @@ EvalOut mrqcof(const Problem& P, const double* a, double conw_r2_rows, pso_order
   for (int c = 1; c <= ma; c++)
     if (P.ia[c]) cmap.push_back(c);
   const int G = tile_chains(row_blocks(P));
-  std::vector<double> acc((size_t)G * K * K, 0.0); /* G chains, lower triangle incl. residual row */
-  std::vector<double> rows((size_t)(P.max_lp > 3 ? P.max_lp : 3) * (ma + 1));
-  std::vector<double> X((size_t)TL * K), Xw((size_t)TL * K);
-  int pt0 = 0, tile = 0;
-  auto syrk_tile = [&](int np, const double* Xt, const double* s2w) {
-    double* A = acc.data() + (size_t)(tile % G) * K * K;
+  /* raw rows of every point, then the convexity rows */
+  std::vector<double> rows((size_t)(P.ndata + 3) * (ma + 1));
+  for (int p = 0; p < P.ndata; p++) out.nvis += point_row(P, v, area, p, &rows[(size_t)p * (ma + 1)]);
+  convexity_rows(P, area, &rows[(size_t)P.ndata * (ma + 1)]);
+  if (jrows_out) std::memcpy(jrows_out, rows.data(), sizeof(double) * rows.size());
+  /* brightness sum of each relative curve: per 16-point tile four partial
+     sums over points p = s (mod 4), (G0+G1)+(G2+G3), then the tile sums in
+     order; kept as its reciprocal */
+  std::vector<int> c0(NC + 1, 0);
+  for (int ci = 0; ci < NC; ci++) c0[ci + 1] = c0[ci] + P.l_points[ci];
+  std::vector<double> rave(NC, 0.0);
+  for (int ci = 0; ci < NC; ci++) {
+    if (!P.in_rel[ci]) continue;
+    const int lp = P.l_points[ci];
+    double s = 0.0;
+    for (int t0 = 0; t0 < lp; t0 += 16) {
+      double g4[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int p = 0; p < 16; p++) {
+        const double val = (t0 + p < lp) ? rows[(size_t)(c0[ci] + t0 + p) * (ma + 1)] : 0.0;
+        g4[p & 3] = g4[p & 3] + val;
+      }
+      s = s + ((g4[0] + g4[1]) + (g4[2] + g4[3]));
+    }
+    rave[ci] = 1.0 / s;
+  }
+  /* G chains: lower triangles of (W Z)^T Z, K2 x K2; per (chain, relative
+     curve) the rows z and y at the end of the chain's tiles of the curve */
+  std::vector<double> acc((size_t)G * K2 * K2, 0.0);
+  std::vector<double> rz((size_t)G * NC * K2, 0.0), ry((size_t)G * NC * K2, 0.0);
+  std::vector<char> seen((size_t)G * NC, 0);
+  std::vector<double> X((size_t)TL * K2, 0.0), XW((size_t)TL * K2, 0.0);
+  /* the columns that carry values (the others stay zero and no element
+     read later depends on them) */
+  std::vector<int> act;
+  for (int q = 0; q <= mf; q++) act.push_back(q);
+  act.push_back(ZC);
+  act.push_back(YC);
+  int tile = 0;
+  auto syrk_tile = [&](int np) {
+    double* A = acc.data() + (size_t)(tile % G) * K2 * K2;
     for (int p = 0; p < np; p++)
-      for (int i = 0; i < K; i++) {
-        const double xi = Xt[p * K + i];
-        for (int j = 0; j <= i; j++) A[i * K + j] = std::fma(xi, s2w[p] * Xt[p * K + j], A[i * K + j]);
+      for (size_t ii = 0; ii < act.size(); ii++) {
+        const int i = act[ii];
+        const double wi = XW[p * K2 + i];
+        for (size_t jj = 0; jj <= ii; jj++) {
+          const int j = act[jj];
+          A[i * K2 + j] = std::fma(wi, X[p * K2 + j], A[i * K2 + j]);
+        }
       }
     tile++;
   };
-  std::vector<double> s2w(TL);
-  for (int ci = 0; ci < P.l_curves; ci++) {
+  for (int ci = 0; ci < NC; ci++) {
     const int lp = P.l_points[ci];
-    for (int j = 0; j < lp; j++) {
-      out.nvis += point_row(P, v, area, pt0 + j, &rows[(size_t)j * (ma + 1)]);
-      if (jrows_out) std::memcpy(jrows_out + (size_t)(pt0 + j) * (ma + 1), &rows[(size_t)j * (ma + 1)], sizeof(double) * (ma + 1));
-    }
-    if (P.in_rel[ci]) {
-      /* curve sums (Start.cu:2454-2531) in tile order: per 16-point tile,
-         partial sums over points p = s (mod 4), (G0+G1)+(G2+G3), then the
-         tile sums in order */
-      std::vector<double> d(ma + 1, 0.0);
-      for (int t0 = 0; t0 < lp; t0 += 16) {
-        for (int c = 0; c <= ma; c++) {
-          if (c == 1) continue;
-          double g4[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int p = 0; p < 16; p++) {
-            const double val = (t0 + p < lp) ? rows[(size_t)(t0 + p) * (ma + 1) + c] : 0.0;
-            g4[p & 3] = g4[p & 3] + val;
-          }
-          d[c] = d[c] + ((g4[0] + g4[1]) + (g4[2] + g4[3]));
+    const bool rel = P.in_rel[ci] != 0;
+    for (int t0 = 0; t0 < lp; t0 += TL) {
+      const int np = lp - t0 < TL ? lp - t0 : TL;
+      const int g = tile % G;
+      for (int p = 0; p < np; p++) {
+        const int gp = c0[ci] + t0 + p;
+        const double* r = &rows[(size_t)gp * (ma + 1)];
+        const double y = r[0];
+        const double coef = rel ? (P.sig[gp] * (double)lp) * rave[ci] : 1.0;
+        const double s2 = P.sigr2[gp] * P.weight[gp];
+        for (int q = 0; q < mf; q++) {
+          const int c = cmap[q];
+          const double xr = (c == 1) ? (rel ? 0.0 : y) : r[c];
+          X[p * K2 + q] = coef * xr;
+          XW[p * K2 + q] = s2 * X[p * K2 + q];
         }
+        const double res = P.bright[gp] - coef * y;
+        X[p * K2 + mf] = res;
+        XW[p * K2 + mf] = s2 * res;
+        X[p * K2 + ZC] = 0.0;
+        XW[p * K2 + ZC] = rel ? 1.0 : 0.0;
+        X[p * K2 + YC] = y;
+        XW[p * K2 + YC] = rel ? s2 * y : 0.0;
       }
-      const double Lsum = d[0];
-      const double rave = 1.0 / Lsum;
-      for (int t0 = 0; t0 < lp; t0 += TL) {
-        int np = lp - t0 < TL ? lp - t0 : TL;
-        for (int p = 0; p < np; p++) {
-          const int j = t0 + p, gp = pt0 + j;
-          const double* r = &rows[(size_t)j * (ma + 1)];
-          const double coef = (P.sig[gp] * (double)lp) * rave;
-          const double coef1 = r[0] * rave;
-          for (int q = 0; q < mf; q++) {
-            const int c = cmap[q];
-            X[p * K + q] = (c == 1) ? 0.0 : coef * std::fma(-coef1, d[c], r[c]);
-          }
-          const double ym = coef * r[0];
-          X[p * K + mf] = P.bright[gp] - ym;
-          s2w[p] = P.sigr2[gp] * P.weight[gp];
+      syrk_tile(np);
+      /* the chain's last tile of this relative curve (its next tile is G
+         tiles on): rows z, y out */
+      if (rel) {
+        if (t0 + G * TL >= lp) {
+          double* A = acc.data() + (size_t)g * K2 * K2;
+          double* oz = &rz[((size_t)g * NC + ci) * K2];
+          double* oy = &ry[((size_t)g * NC + ci) * K2];
+          for (int j = 0; j <= ZC; j++) oz[j] = A[ZC * K2 + j];
+          for (int j = 0; j <= YC; j++) oy[j] = A[YC * K2 + j];
+          for (int j = 0; j <= ZC; j++) A[ZC * K2 + j] = 0.0;
+          for (int j = 0; j <= YC; j++) A[YC * K2 + j] = 0.0;
+          seen[(size_t)g * NC + ci] = 1;
         }
-        syrk_tile(np, X.data(), s2w.data());
-      }
-    } else {
-      /* absolute curve: no renormalisation (Start.cu:2656-2715, RENORM=0) */
-      for (int t0 = 0; t0 < lp; t0 += TL) {
-        int np = lp - t0 < TL ? lp - t0 : TL;
-        for (int p = 0; p < np; p++) {
-          const int j = t0 + p, gp = pt0 + j;
-          const double* r = &rows[(size_t)j * (ma + 1)];
-          for (int q = 0; q < mf; q++) X[p * K + q] = r[cmap[q]];
-          X[p * K + mf] = P.bright[gp] - r[0];
-          s2w[p] = P.sigr2[gp] * P.weight[gp];
-        }
-        syrk_tile(np, X.data(), s2w.data());
       }
     }
-    pt0 += lp;
   }
   /* convexity pseudo-curve: 3 rows, brightness 0, sigr2 = conwR^2, weight 1 */
   {
-    std::vector<double> cr(3 * (ma + 1));
-    convexity_rows(P, area, cr.data());
-    if (jrows_out) std::memcpy(jrows_out + (size_t)P.ndata * (ma + 1), cr.data(), sizeof(double) * 3 * (ma + 1));
+    const double* cr = &rows[(size_t)P.ndata * (ma + 1)];
     for (int p = 0; p < 3; p++) {
       const double* r = &cr[p * (ma + 1)];
-      for (int q = 0; q < mf; q++) X[p * K + q] = r[cmap[q]];
-      X[p * K + mf] = 0.0 - r[0];
-      s2w[p] = conw_r2_rows * 1.0;
+      const double s2 = conw_r2_rows * 1.0;
+      for (int q = 0; q < mf; q++) {
+        X[p * K2 + q] = (cmap[q] == 1) ? r[0] : r[cmap[q]];
+        XW[p * K2 + q] = s2 * X[p * K2 + q];
+      }
+      X[p * K2 + mf] = 0.0 - r[0];
+      XW[p * K2 + mf] = s2 * X[p * K2 + mf];
+      X[p * K2 + ZC] = XW[p * K2 + ZC] = 0.0;
+      X[p * K2 + YC] = XW[p * K2 + YC] = 0.0;
     }
-    syrk_tile(3, X.data(), s2w.data());
+    syrk_tile(3);
   }
+  /* the chains' partial matrices, (C0+C1)+(C2+C3) (or C0+C1) */
   std::vector<double> tot((size_t)K * K);
-  for (size_t e = 0; e < (size_t)K * K; e++) {
-    const double* c0 = acc.data() + e;
-    const size_t KK = (size_t)K * K;
-    tot[e] = (G == 4) ? (c0[0] + c0[KK]) + (c0[2 * KK] + c0[3 * KK]) : (G == 2) ? c0[0] + c0[KK] : c0[0];
+  const size_t KK = (size_t)K2 * K2;
+  for (int i = 0; i < K; i++)
+    for (int j = 0; j <= i; j++) {
+      const double* a0 = acc.data() + (size_t)i * K2 + j;
+      tot[(size_t)i * K + j] = (G == 4)   ? (a0[0] + a0[KK]) + (a0[2 * KK] + a0[3 * KK])
+                               : (G == 2) ? a0[0] + a0[KK]
+                                          : a0[0];
+    }
+  /* per relative curve: the chains' rows combined the same way (a chain
+     without a tile of the curve gives 0.0); correction vectors
+     a = U - (Y / 2L) D (a[mf] = R), b = D / L (b[mf] = 0), zero for
+     absolute curves and for the padding curve that makes the count even */
+  const int NCP = (NC + 1) & ~1;
+  std::vector<double> T((size_t)2 * NCP * K, 0.0);
+  auto comb = [&](const std::vector<double>& src, int ci, int j) {
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < G; g++)
+      if (seen[(size_t)g * NC + ci]) x[g] = src[((size_t)g * NC + ci) * K2 + j];
+    return (G == 4) ? (x[0] + x[1]) + (x[2] + x[3]) : x[0] + x[1];
+  };
+  for (int ci = 0; ci < NC; ci++) {
+    if (!P.in_rel[ci]) continue;
+    const double rv = rave[ci];
+    const double Y = comb(ry, ci, YC);
+    double* ta = &T[(size_t)(2 * ci) * K];
+    double* tb = &T[(size_t)(2 * ci + 1) * K];
+    for (int q = 0; q < mf; q++) {
+      const double D = comb(rz, ci, q), U = comb(ry, ci, q);
+      ta[q] = std::fma(-(0.5 * (Y * rv)), D, U);
+      tb[q] = rv * D;
+    }
+    ta[mf] = comb(ry, ci, mf);
+    tb[mf] = 0.0;
   }
+  for (int i = 0; i < K; i++)
+    for (int j = 0; j <= i; j++) {
+      double t = tot[(size_t)i * K + j];
+      for (int k2 = 0; k2 < 2 * NCP; k2++) t = std::fma(-T[(size_t)k2 * K + i], T[(size_t)(k2 ^ 1) * K + j], t);
+      tot[(size_t)i * K + j] = t;
+    }
   for (int i = 0; i < mf; i++)
     for (int j = 0; j <= i; j++) {
       alpha[i * mf + j] = tot[i * K + j];

@@ -201,24 +201,23 @@ int check_problem(ps_ctx* c, const ps_problem* p, int* mfit_out) {
   int count = 0;
   for (int i = 0; i < L; ++i) {
     if (p->l_points[i] <= 0) return fail(c, PS_E_ARG, "empty lightcurve");
-    if (!p->in_rel[i]) return fail(c, PS_E_UNSUPPORTED, "calibrated (absolute) lightcurves are not supported");
     count += p->l_points[i];
   }
   if (count != N) return fail(c, PS_E_ARG, "l_points do not sum to ndata");
   const int ma = C + 8;
-  // free parameters must be one contiguous run starting at parameter 2
+  // free parameters must be one contiguous run starting at parameter 2, or
+  // at parameter 1 (the size scale, free with absolute data)
   int mfit = 0, lastma = 0;
   for (int i = 0; i < ma; ++i)
     if (p->ia[i]) {
       ++mfit;
       lastma = i + 1;
     }
-  if (p->ia[0]) return fail(c, PS_E_UNSUPPORTED, "the size-scale parameter must be fixed (relative data)");
   for (int i = 1; i < lastma; ++i)
-    if (!p->ia[i]) return fail(c, PS_E_UNSUPPORTED, "free parameters must be contiguous from parameter 2");
+    if (!p->ia[i])
+      return fail(c, PS_E_UNSUPPORTED, "free parameters must be contiguous from parameter 2 (or 1)");
   if (mfit < 1) return fail(c, PS_E_ARG, "no free parameters");
-  const int K = mfit + 1;
-  const int nb = std::max((C + 7 + 15) / 16, (K + 15) / 16);
+  const int nb = ps_row_blocks(C, mfit);
   if (nb > 6 || (C - 1 + 15) / 16 < nb - 1)
     return fail(c, PS_E_UNSUPPORTED, "more than 96 Jacobian columns (lmax too large)");
   *mfit_out = mfit;
@@ -258,8 +257,8 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
     if (i > 0 && (rc = check_shared(c, ps_[0], ps_[i])) != PS_OK) return rc;
   }
   const ps_problem* p0 = ps_[0];
-  const int F = p0->nfac, C = p0->ncoef, ma = C + 8, K = mfit + 1;
-  const int nb = std::max((C + 7 + 15) / 16, (K + 15) / 16);
+  const int F = p0->nfac, C = p0->ncoef, ma = C + 8;
+  const int nb = ps_row_blocks(C, mfit);
   const int nbs = (C - 1 + 15) / 16;
   PsDevProblem shared{};
   int rc;
@@ -294,6 +293,7 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
   shared.nb = nb;
   shared.qs = 16 * nb;
   shared.nbs = nbs;
+  shared.ia1 = p0->ia[0] ? 1 : 0;
   c->stride = 0;
   for (int pi = 0; pi < n; ++pi) {
     const ps_problem* p = ps_[pi];
@@ -315,7 +315,9 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
       for (int t = 0; t < nt; ++t) {
         tcv.push_back(i);
         const int n_in = std::min(16, p->l_points[i] - 16 * t);
-        tinfo.insert(tinfo.end(), {off[i] + 16 * t, n_in, i, p->l_points[i]});
+        // the curve length carries the curve's kind: negative for an
+        // absolute (calibrated) curve
+        tinfo.insert(tinfo.end(), {off[i] + 16 * t, n_in, i, p->in_rel[i] ? p->l_points[i] : -p->l_points[i]});
       }
     }
     UP(d.tim, p->tim, N);

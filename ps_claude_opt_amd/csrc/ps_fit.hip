@@ -19,24 +19,32 @@
 //                 point of the tile sees any of the 4 facets are skipped;
 //                 sorting by view direction makes such steps common.  Raw
 //                 Jacobian rows go to the workgroup's scratch slab.
-//   tile sums     column sums of each 16-point tile of a curve, from the rows
-//   curve sums    column sums d and brightness sum L of each curve, from
-//                 its tile sums
-//   phase 2       relative renormalisation J' = c (J - y d / L), residual
-//                 r; the augmented product [J'|r]^T W [J'|r] is an FP64
-//                 MFMA SYRK split into tile chains, each wave owning lower
-//                 16x16 block pairs of one chain in registers: it gives
-//                 alpha, beta and chi^2 at once.
+//   curve sums    brightness sum L of each relative curve (from the model
+//                 brightness alone) and its factor c = sig lp / L
+//   phase 2       one read of the rows: the augmented product
+//                 [cJ|r]^T W [cJ|r] (r = obs - c y) is an FP64 MFMA SYRK
+//                 split into tile chains, each wave owning lower 16x16
+//                 block pairs of one chain in registers; per relative curve
+//                 the chain also sums d = sum J, u = sum w y J, sum w y^2
+//                 and sum w y r
+//   correction    the relative renormalisation J' = c (J - y d / L)
+//                 (Start.cu:2656-2715) applied algebraically: a rank-2 term
+//                 per curve, as one MFMA chain per block pair on the
+//                 combined matrix.  Alpha, beta and chi^2 come out at once.
 // mrqmin: full-pivot Gauss-Jordan in LDS by all waves, accept / reject,
 // Iter2, FinishPole (Start.cu:2838-3021, 300-392, 5129-5248, 5056-5125).
 //
 // Operation order is fixed (see the CPU oracle): facet partial sums by
 // facet mod 4 combined (P0+P1)+(P2+P3); the MFMA gather is a fused
-// multiply-add chain over facets in ascending order; tile column sums are
-// four partials over points p mod 4 combined (G0+G1)+(G2+G3), curve sums add
-// the tile sums in order; each element of the augmented normal matrix is
+// multiply-add chain over facets in ascending order; tile brightness sums
+// are four partials over points p mod 4 combined (G0+G1)+(G2+G3), curve sums
+// add the tile sums in order; a chain's per-curve sums are per-lane chains
+// over its tiles of the curve combined (S0+S1)+(S2+S3), the chains as the
+// matrices; each element of the augmented normal matrix is
 // the combination (C0+C1)+(C2+C3) (or C0+C1 above 64 columns) of fused
-// multiply-add chains over the tiles T = g (mod chains), points in order.
+// multiply-add chains over the tiles T = g (mod chains), points in order,
+// followed by one fused multiply-add chain of the renormalisation
+// correction over the curves in order.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -276,7 +284,7 @@ struct Lds {
 
 // Pointers into one workgroup's scratch slab.
 struct Slab {
-  double *jbuf, *ybuf, *tsum, *dsum, *part, *best, *prec;
+  double *jbuf, *ybuf, *tsy, *crv, *csum, *ttab, *part, *best, *prec;
   int* perm;
   unsigned char* pkey;
 };
@@ -286,8 +294,10 @@ __device__ __forceinline__ Slab make_slab(const PsDevProblem& P, double* base) {
   Slab s;
   s.jbuf = base + o.jbuf;
   s.ybuf = base + o.ybuf;
-  s.tsum = base + o.tsum;
-  s.dsum = base + o.dsum;
+  s.tsy = base + o.tsy;
+  s.crv = base + o.crv;
+  s.csum = base + o.csum;
+  s.ttab = base + o.ttab;
   s.part = base + o.part;
   s.best = base + o.best;
   s.prec = base + o.prec;
@@ -577,86 +587,19 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   return lane == 0 ? nvis : 0u;
 }
 
-// Column sums of the data tiles (16 consecutive points of one curve,
-// aligned at the curve start) from the rows in the slab: lane (ps, sl) sums
-// column 16b + ps of the points 4r + sl in order, the lane groups combine as
-// (G0 + G1) + (G2 + G3); padding rows add +0.0.  Column qs: brightness.
-template <int NB>
-struct TileRows {
-  double v[4][NB], y[4];
-  int n;  // points in the tile
-};
-
-template <int NB>
-__device__ __forceinline__ void tile_rows_load(const PsDevProblem& P, const Slab& S, int4 ti, TileRows<NB>& tr) {
-  const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
-  const int QS = P.qs;
-  tr.n = ti.y;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int ptl = 4 * r + sl;
-    const int prow = ti.x + (ptl < ti.y ? ptl : 0);
-    const double* jr = S.jbuf + (size_t)prow * QS;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) tr.v[r][b] = jr[16 * b + ps];
-    tr.y[r] = S.ybuf[prow];
-  }
-}
-
-template <int NB>
-__device__ __forceinline__ void tile_rows_sum(const PsDevProblem& P, const Slab& S, int T, const TileRows<NB>& tr) {
-  const int lane = lane_id(), ps = lane & 15, sl = lane >> 4;
-  const int QS = P.qs;
-  double tsq[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) tsq[b] = 0.0;
-  double tsy = 0.0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const bool ok = 4 * r + sl < tr.n;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) tsq[b] = tsq[b] + (ok ? tr.v[r][b] : 0.0);
-    tsy = tsy + (ok ? tr.y[r] : 0.0);
-  }
-#pragma unroll
-  for (int b = 0; b < NB; ++b) tsq[b] = xsum16_32(tsq[b]);
-  tsy = xsum16_32(tsy);
-  double* ts = S.tsum + (size_t)T * (QS + 1);
-  if (sl == 0) {
-#pragma unroll
-    for (int b = 0; b < NB; ++b) ts[16 * b + ps] = tsq[b];
-    if (ps == 0) ts[QS] = tsy;
-  }
-}
-
-// Wave w's data tiles T = w (mod NW), TT at a time: every row load of a
-// group is issued before its first sum, and the next group's tile
-// descriptors are loaded while the rows are in flight.
-template <int NB>
-__device__ __forceinline__ void tile_sums(const PsDevProblem& P, const Slab& S, int w) {
-  constexpr int TT = NB <= 4 ? 3 : 2;  // tiles in flight per wave
-  const int nt = P.ntiles;
-  auto info = [&](int T) __attribute__((always_inline)) {
-    return T < nt ? *reinterpret_cast<const int4*>(P.tile_info + 4 * T) : make_int4(0, 0, 0, 0);
-  };
-  int4 ti[TT];
-#pragma unroll
-  for (int k = 0; k < TT; ++k) ti[k] = info(w + k * NW);
-  for (int T = w; T < nt; T += TT * NW) {
-    TileRows<NB> tr[TT];
-#pragma unroll
-    for (int k = 0; k < TT; ++k)
-      if (T + k * NW < nt) tile_rows_load<NB>(P, S, ti[k], tr[k]);
-#pragma unroll
-    for (int k = 0; k < TT; ++k) ti[k] = info(T + (TT + k) * NW);
-#pragma unroll
-    for (int k = 0; k < TT; ++k)
-      if (T + k * NW < nt) tile_rows_sum<NB>(P, S, T + k * NW, tr[k]);
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Phase 2: the augmented normal matrix [J'|r]^T W [J'|r] on FP64 MFMA.
+// Phase 2: the augmented normal matrix on FP64 MFMA.  Relative curves are
+// renormalised algebraically (Start.cu:2656-2715 restated): the chains
+// accumulate (W Z)^T Z for rows Z = [c J | r | ... | z | y] (r = obs - c y,
+// c = sig lp / L; z and y are the last two columns of the last block)
+// whose weighted side is [w c J | w r | ... | 1 | w y] on relative curves
+// and 0 in the two sum columns otherwise.  Row z of the product then holds
+// D = sum c J, row y holds U = sum w y c J,
+// Y = sum w y^2 and R = sum w y r: at the end of each relative curve the
+// chain writes those two rows out and clears them.  The correction
+// -(a_k b_k^T + b_k a_k^T) per curve (a = U - (Y / 2L) D with a[mf] = R,
+// b = D / L) is applied afterwards, so the rows are read once and no
+// per-curve pass over them precedes the product.
 // The waves form P2<NB>::GT point chains; chain g accumulates the tiles
 // T = g (mod GT) (data tiles in curve order, then the convexity tile) and,
 // for NB > 4, its two waves split the 16x16 block pairs (ib >= jb).  Every
@@ -673,12 +616,15 @@ struct P2 {
 };
 static_assert(P2<4>::GT == 4 && P2<5>::GT == 2, "chain count must match ps_tile_chains");
 
-// Phase 2 rows: lane (ps, sl) handles column 16b + ps of the points 4r + sl
-// (r = 0..3) of a tile.  The column's kind is fixed per lane and block: a
-// Jacobian column (q < mfit), the residual (q == mfit) or padding.
+// Phase 2 rows: lane (ps, sl) handles compact column 16b + ps of the points
+// 4r + sl (r = 0..3) of a tile.  The column's kind is fixed per lane and
+// block: a Jacobian column (q < mfit), the residual (q == mfit), padding,
+// or in the last block the sum column z (q = 16 NB - 2) and the brightness
+// column y (q = 16 NB - 1).
 template <int NB>
 struct P2Cols {
-  unsigned jm = 0, rm = 0;  // bit b: block b's column is a Jacobian column / the residual
+  unsigned jm = 0, rm = 0;
+  bool z = false, y = false;  // the lane's last-block column is z / y
   __device__ __forceinline__ P2Cols(int ps, int mf) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -686,10 +632,13 @@ struct P2Cols {
       if (q < mf) jm |= 1u << b;
       if (q == mf) rm |= 1u << b;
     }
+    z = ps == 14;
+    y = ps == 15;
   }
 };
 
-// The products of one row into the wave's block pairs.
+// The products of one row into the wave's block pairs: element (i, j)
+// accumulates (weighted side)_i * (plain side)_j.
 template <int NB, int NPW, int GB>
 __device__ __forceinline__ void p2_products(const double* X, const double* XW, int gb, v4d* acc) {
   int idx = 0;
@@ -697,7 +646,7 @@ __device__ __forceinline__ void p2_products(const double* X, const double* XW, i
   for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
     for (int jb = 0; jb <= ib; ++jb) {
-      if (idx % GB == gb) acc[idx / GB] = mfma(X[ib], XW[jb], acc[idx / GB]);
+      if (idx % GB == gb) acc[idx / GB] = mfma(XW[ib], X[jb], acc[idx / GB]);
       ++idx;
     }
 }
@@ -705,14 +654,21 @@ __device__ __forceinline__ void p2_products(const double* X, const double* XW, i
 // One wave's share of phase 2: its chain's data tiles, the next row's loads
 // issued before the current row's products, then (on the chain that owns
 // it) the convexity tile, which closes the chain.  Writes the chain's
-// partial matrix.
+// partial matrix and, at the end of each relative curve, rows z and y of
+// that curve.  Compact column q is parameter q + 2 - ia1, stored in the raw
+// row at column q - ia1; with ia1 the lane of compact column 0 holds the
+// size scale, whose derivative is y (zero after renormalisation).
 template <int NB>
 __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w) {
   using SH = P2<NB>;
   const int tid = tid_opaque(), lane = tid & 63, ps = lane & 15, sl = lane >> 4, w = tid >> 6;
   const int gt = w / SH::GB, gb = w % SH::GB;
-  const int mf = P.mfit, C = P.ncoef, QS = P.qs;
+  const int mf = P.mfit, C = P.ncoef, QS = P.qs, ia1 = P.ia1;
+  const int CWS = ps_curve_sums_width(QS);
   const P2Cols<NB> K2(ps, mf);
+  const bool p1 = ia1 != 0 && ps == 0;  // block 0 of this lane is the size scale
+  const int c0 = ps - ia1 < 0 ? 0 : ps - ia1;  // row column of block 0 (any valid one for the size scale)
+  const int c1 = ps - ia1 + 16;                // row column of block 1; block b: c1 + 16 (b - 1)
   v4d acc[SH::NPW];
 #pragma unroll
   for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -720,13 +676,28 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
     double J[NB];
     double y, sig, obs, s2;
   };
+  // with fl: rows z and y (16 NB - 2, 16 NB - 1: block row NB - 1, r = 3,
+  // lanes sl = 2, 3) of this chain's matrix out for curve c, then cleared
+  auto flush = [&](int c, bool fl) __attribute__((always_inline)) {
+    double* o = S.csum + ((size_t)gt * P.ncurves + c) * CWS + (sl == 2 ? 0 : QS) + ps;
+#pragma unroll
+    for (int jb = 0; jb < NB; ++jb) {
+      constexpr int base = (NB - 1) * NB / 2;
+      const int idx = base + jb;
+      if (idx % SH::GB == gb) {
+        if (fl && sl >= 2) o[16 * jb] = acc[idx / SH::GB][3];
+        acc[idx / SH::GB][3] = (fl && sl >= 2) ? 0.0 : acc[idx / SH::GB][3];
+      }
+    }
+  };
   // ALLJ: blocks 0 .. NB-2 hold Jacobian columns only (16 (NB - 1) <= mfit,
   // true for every input the triangulation and basis produce); only the
   // last block then needs the per-lane column kind.
   auto tiles = [&](auto allj_tag) __attribute__((always_inline)) {
     constexpr bool ALLJ = decltype(allj_tag)::value;
-    // first point, points in the tile, curve, curve length (one load);
-    // past the last tile: a one-point tile at point 0
+    // first point, points in the tile, curve, curve length (negative:
+    // absolute curve), one load; past the last tile: a one-point tile at
+    // point 0
     auto info = [&](int T) __attribute__((always_inline)) {
       return T < P.ntiles ? *reinterpret_cast<const int4*>(P.tile_info + 4 * T) : make_int4(0, 1, 0, 1);
     };
@@ -734,9 +705,10 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
       Row R;
       const int pin = 4 * r + sl;
       const int pt = t.x + (pin < t.y ? pin : 0);
-      const double* jr = S.jbuf + (size_t)pt * QS + ps;
+      const double* jr = S.jbuf + (size_t)pt * QS;
+      R.J[0] = jr[c0];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) R.J[b] = jr[16 * b];
+      for (int b = 1; b < NB; ++b) R.J[b] = jr[c1 + 16 * (b - 1)];
       R.y = S.ybuf[pt];
       R.sig = P.sig[pt];
       R.obs = P.bright[pt];
@@ -747,16 +719,15 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
     Row nxt = load(ti, 0);
     for (int T = gt; T < P.ntiles; T += SH::GT) {
       const int4 tn = info(T + SH::GT);
-      const int n_in = ti.y, c = ti.z, lp = ti.w;
-      const double* ds = S.dsum + (size_t)c * (QS + 1);
-      double dv[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) dv[b] = ds[16 * b + ps];
-      const double rave = ds[QS];
-      const double lpd = (double)lp;
-      // FULL: the tile has 16 points (every tile but a curve's last one)
-      auto rows = [&](auto full_tag) __attribute__((always_inline)) {
+      const int n_in = ti.y, c = ti.z;
+      const bool rel = ti.w > 0;
+      const double lpd = (double)(rel ? ti.w : -ti.w);
+      const double rave = rel ? S.crv[c] : 0.0;
+      // FULL: the tile has 16 points (every tile but a curve's last one);
+      // REL: a relative curve
+      auto rows = [&](auto full_tag, auto rel_tag) __attribute__((always_inline)) {
         constexpr bool FULL = decltype(full_tag)::value;
+        constexpr bool REL = decltype(rel_tag)::value;
         Row rw[2];
         rw[0] = nxt;
 #pragma unroll
@@ -767,31 +738,48 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
             nxt = load(tn, 0);
           const Row& cur = rw[r & 1];
           const bool ok = FULL || 4 * r + sl < n_in;
-          // relative renormalisation (Start.cu:2656-2715)
-          const double coef = (cur.sig * lpd) * rave;
-          const double coef1 = cur.y * rave;
+          const double coef = REL ? (cur.sig * lpd) * rave : 1.0;
           const double res = cur.obs - coef * cur.y;
           const double s2v = ok ? cur.s2 : 0.0;
           double X[NB], XW[NB];
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            const double xj = coef * fma(-coef1, dv[b], cur.J[b]);
-            double xv;
-            if (ALLJ && b < NB - 1)
-              xv = xj;
-            else
-              xv = (K2.jm >> b) & 1u ? xj : (K2.rm >> b) & 1u ? res : 0.0;
-            xv = ok ? xv : 0.0;
+            double xr = cur.J[b];
+            if (b == 0) xr = p1 ? (REL ? 0.0 : cur.y) : xr;
+            const double xj = coef * xr;
+            double xv, xw;
+            if (ALLJ && b < NB - 1) {
+              xv = ok ? xj : 0.0;
+              xw = s2v * xv;
+            } else {
+              const bool isj = (K2.jm >> b) & 1u, isr = (K2.rm >> b) & 1u;
+              const bool isz = b == NB - 1 && K2.z, isy = b == NB - 1 && K2.y;
+              xv = isj ? xj : isr ? res : isy ? cur.y : 0.0;
+              xv = ok ? xv : 0.0;
+              xw = s2v * xv;
+              if (isz) xw = (REL && ok) ? 1.0 : 0.0;
+              if (isy) xw = REL ? s2v * cur.y : 0.0;
+            }
             X[b] = xv;
-            XW[b] = s2v * xv;
+            XW[b] = xw;
           }
           p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
         }
       };
-      if (n_in == 16)
-        rows(std::true_type{});
-      else
-        rows(std::false_type{});
+      if (rel) {
+        if (n_in == 16)
+          rows(std::true_type{}, std::true_type{});
+        else
+          rows(std::false_type{}, std::true_type{});
+        // the chain's last tile of this curve (predicated, not branched
+        // around: a branch here costs registers in the tile loop)
+        flush(c, T + SH::GT >= P.ntiles || tn.z != c);
+      } else {
+        if (n_in == 16)
+          rows(std::true_type{}, std::false_type{});
+        else
+          rows(std::false_type{}, std::false_type{});
+      }
       ti = tn;
     }
   };
@@ -801,18 +789,20 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
     tiles(std::false_type{});
   // the convexity pseudo-curve (rows ndata .. ndata+2, Start.cu:2539-2612):
   // tile T = ntiles, the last tile of chain ntiles mod GT; only its first
-  // row group holds rows (lanes sl < 3)
+  // row group holds rows (lanes sl < 3); raw rows, no sums
   if (P.ntiles % SH::GT == gt) {
     const bool ok = sl < 3;
     const int pt = P.ndata + (ok ? sl : 0);
-    const double* jr = S.jbuf + (size_t)pt * QS + ps;
-    const double res = 0.0 - S.ybuf[pt];
+    const double* jr = S.jbuf + (size_t)pt * QS;
+    const double yv = S.ybuf[pt];
+    const double res = 0.0 - yv;
     const double s2v = ok ? conw_w * 1.0 : 0.0;
     double X[NB], XW[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int q = 16 * b + ps;
-      double xv = (K2.jm >> b) & 1u ? (q + 2 <= C ? jr[16 * b] : 0.0) : (K2.rm >> b) & 1u ? res : 0.0;
+      const int par = 16 * b + ps + 2 - ia1;  // parameter of the lane's column (1-based)
+      const double jv = b == 0 ? jr[c0] : jr[c1 + 16 * (b - 1)];
+      double xv = (K2.jm >> b) & 1u ? (par == 1 ? yv : par <= C ? jv : 0.0) : (K2.rm >> b) & 1u ? res : 0.0;
       xv = ok ? xv : 0.0;
       X[b] = xv;
       XW[b] = s2v * xv;
@@ -829,6 +819,109 @@ __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& 
       if (idx % SH::GB == gb) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) pt[(size_t)(16 * ib + 4 * r + sl) * ld + 16 * jb + ps] = acc[idx / SH::GB][r];
+      }
+      ++idx;
+    }
+}
+
+// Brightness sums of the data tiles (16 consecutive points of one curve, in
+// file order: four partials over points p mod 4, (G0 + G1) + (G2 + G3)),
+// then per relative curve 1 / L, L = the curve's tile sums in order.
+__device__ __forceinline__ void curve_sums(const PsDevProblem& P, const Slab& S) {
+  const int tid = tid_opaque();
+  for (int T = tid; T < P.ntiles; T += WG) {
+    const int4 ti = *reinterpret_cast<const int4*>(P.tile_info + 4 * T);
+    double g4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+      const double val = p < ti.y ? S.ybuf[ti.x + p] : 0.0;
+      g4[p & 3] = g4[p & 3] + val;
+    }
+    S.tsy[T] = (g4[0] + g4[1]) + (g4[2] + g4[3]);
+  }
+  __syncthreads();
+  for (int c = tid; c < P.ncurves; c += WG) {
+    const int t0 = P.tile_off[c], t1 = P.tile_off[c + 1];
+    if (P.tile_info[4 * t0 + 3] <= 0) continue;  // absolute curve
+    double s = 0.0;
+    for (int T = t0; T < t1; ++T) s = s + S.tsy[T];
+    S.crv[c] = 1.0 / s;
+  }
+}
+
+// The correction vectors of every curve (zero for absolute curves and the
+// padding curve): the chains' rows z (D) and y (U) of the curve combined as
+// the matrices are (a chain without a tile of the curve gives 0.0), then
+// a = U - (Y / 2L) D with a[mf] = R, b = D / L with b[mf] = 0, where
+// Y = U[16 NB - 1] and R = U[mf].
+template <int NB>
+__device__ __forceinline__ void correction_table(const PsDevProblem& P, const Slab& S) {
+  using SH = P2<NB>;
+  const int QS = P.qs, mf = P.mfit, NCP = ps_corr_curves(P.ncurves), CWS = ps_curve_sums_width(QS);
+  auto comb = [](const double* x) { return SH::GT == 4 ? (x[0] + x[1]) + (x[2] + x[3]) : x[0] + x[1]; };
+  for (int e = tid_opaque(); e < NCP * QS; e += WG) {
+    const int c = e / QS, q = e - c * QS;
+    double av = 0.0, bv = 0.0;
+    if (c < P.ncurves && q <= mf && P.tile_info[4 * P.tile_off[c] + 3] > 0) {
+      const int t0 = P.tile_off[c], nt = P.tile_off[c + 1] - t0;
+      double dd[SH::GT], uu[SH::GT], yy[SH::GT];
+#pragma unroll
+      for (int g = 0; g < SH::GT; ++g) {
+        const bool seen = nt >= SH::GT || ((g - t0) % SH::GT + SH::GT) % SH::GT < nt;
+        const double* o = S.csum + ((size_t)g * P.ncurves + c) * CWS;
+        dd[g] = seen ? o[q] : 0.0;
+        uu[g] = seen ? o[QS + q] : 0.0;
+        yy[g] = seen ? o[QS + QS - 1] : 0.0;
+      }
+      const double D = comb(dd), U = comb(uu), Y = comb(yy);
+      const double rave = S.crv[c];
+      if (q < mf) {
+        av = fma(-(0.5 * (Y * rave)), D, U);
+        bv = rave * D;
+      } else {
+        av = U;  // R = sum w y r
+      }
+    }
+    S.ttab[(size_t)(2 * c) * QS + q] = av;
+    S.ttab[(size_t)(2 * c + 1) * QS + q] = bv;
+  }
+}
+
+// The chains' partial matrices combined, (C0 + C1) + (C2 + C3) or C0 + C1,
+// then the correction as one MFMA chain per block pair: element (i, j)
+// accumulates -a_k[i] b_k[j] and then -b_k[i] a_k[j] for every curve k in
+// order.  Wave w takes the lower block pairs idx = w (mod NW).
+template <int NB>
+__device__ __forceinline__ void combine_correct(const PsDevProblem& P, const Slab& S, double* work) {
+  using SH = P2<NB>;
+  const int tid = tid_opaque(), lane = tid & 63, ps = lane & 15, sl = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int K = P.mfit + 1, QS = P.qs, nk = 2 * ps_corr_curves(P.ncurves);
+  const int ld = 16 * NB;
+  const size_t pst = (size_t)256 * NB * NB;
+  int idx = 0;
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int jb = 0; jb <= ib; ++jb) {
+      if (idx % NW == w) {
+        v4d acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double* c0 = S.part + (size_t)(16 * ib + 4 * r + sl) * ld + 16 * jb + ps;
+          acc[r] = SH::GT == 4 ? (c0[0] + c0[pst]) + (c0[2 * pst] + c0[3 * pst]) : c0[0] + c0[pst];
+        }
+        const double* ta = S.ttab + 16 * ib + ps;
+        const double* tb = S.ttab + 16 * jb + ps;
+        for (int k4 = 0; k4 < nk; k4 += 4) {
+          const int k = k4 + sl;
+          acc = mfma(-ta[(size_t)k * QS], tb[(size_t)(k ^ 1) * QS], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * ib + 4 * r + sl, col = 16 * jb + ps;
+          if (row < K && col < K) work[row * K + col] = acc[r];
+        }
       }
       ++idx;
     }
@@ -979,35 +1072,17 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   }
   PS_TMARK(1);
   __syncthreads();
-  // column sums of the data tiles, in file order
-  tile_sums<NB>(P, S, w);
+  // per relative curve: brightness sum and renormalisation factor
+  curve_sums(P, S);
   __syncthreads();
   PS_TMARK(2);
-  // curve sums: the tile sums of each curve in order (column qs: brightness,
-  // stored as its reciprocal)
-  for (int e = tid; e < P.ncurves * K; e += WG) {
-    const int c = e / K, q = e - c * K;
-    const int col = q < mf ? q : QS;
-    double s = 0.0;
-    for (int T = P.tile_off[c]; T < P.tile_off[c + 1]; ++T) s = s + S.tsum[(size_t)T * (QS + 1) + col];
-    S.dsum[(size_t)c * (QS + 1) + col] = q < mf ? s : 1.0 / s;
-  }
-  __syncthreads();
-  PS_TMARK(3);
   phase2_chain<NB>(P, S, conw_r2);
+  __syncthreads();
   PS_TMARK(4);
+  correction_table<NB>(P, S);
   __syncthreads();
   PS_TMARK(5);
-  {
-    const int ld = 16 * NB;
-    const size_t ps_ = (size_t)256 * NB * NB;
-    for (int e = tid; e < K * K; e += WG) {
-      const int i = e / K, j = e - i * K;
-      if (j > i) continue;
-      const double* c0 = S.part + (size_t)i * ld + j;
-      L.work[e] = SH::GT == 4 ? (c0[0] + c0[ps_]) + (c0[2 * ps_] + c0[3 * ps_]) : c0[0] + c0[ps_];
-    }
-  }
+  combine_correct<NB>(P, S, L.work);
   __syncthreads();
   for (int e = tid; e < mf * mf; e += WG) {
     const int i = e / mf, j = e - i * mf;
@@ -1461,15 +1536,14 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
   for (int f = tid; f < P.nfac; f += WG) out_area[f] = L.nrm[4 * f + 3];
   if (out_j != nullptr) {
     // raw rows in the reference's layout: column 0 = model brightness,
-    // column c = d/d(parameter c) (column 1, the size scale, is not computed)
+    // column c = d/d(parameter c); column 1, the size scale, is the model
+    // brightness itself (every area carries the factor exp(a_1))
     const int W = P.ma + 1;
     for (int e = tid; e < (P.ndata + 3) * W; e += WG) {
       const int j = e / W, col = e - j * W;
       double val;
-      if (col == 0)
+      if (col <= 1)
         val = SL.ybuf[j];
-      else if (col == 1)
-        val = 0.0;
       else
         val = (col - 2 < P.qs) ? SL.jbuf[(size_t)j * P.qs + (col - 2)] : 0.0;
       out_j[e] = val;

@@ -20,6 +20,7 @@ struct PsDevProblem {
   const int* tile_off;  // [ncurves+1] first tile of each curve
   const int* tile_cv;   // [ntiles] curve of each tile
   const int* tile_info; // [ntiles][4] first point, points in the tile (1..16), curve, curve length
+                        // (negative for an absolute curve, in_rel = 0)
   int ntiles;
   // facets and basis
   int nfac, lmax, mmax, ncoef, ma, mfit;
@@ -32,6 +33,8 @@ struct PsDevProblem {
                                  // [f][r][b]; the padding facets are zero
   const double* cg_first;        // [ncoef]
   const int* cmap;               // [mfit] compact index -> parameter index (0-based)
+  int ia1;                       // parameter 1 (the size scale) is free: compact column 0 is parameter 1,
+                                 // compact column q >= 1 parameter q + 1 (else compact q is parameter q + 2)
   // model and LM controls
   double par[3], lcl, phi0;
   double alamda_start, alamda_incr, alamda_incrr;
@@ -67,6 +70,14 @@ struct PsFitArgs {
   int batch;            // 1: items of several problems (read from probs), 0: every item is on P0
 };
 
+// 16-column blocks of a row: the raw Jacobian row holds parameters 2..C+8
+// (C + 7 columns); the normal-equation rows hold mfit Jacobian columns, the
+// residual and two columns that carry the per-curve sums.
+__host__ __device__ inline int ps_row_blocks(int ncoef, int mfit) {
+  const int a = (ncoef + 7 + 15) / 16, b = (mfit + 3 + 15) / 16;
+  return a > b ? a : b;
+}
+
 // Point chains of the normal-equation product: the waves of a workgroup
 // form ps_tile_chains(nb) groups; group g accumulates the tiles T = g (mod
 // groups) and the groups' partial matrices are added at the end.
@@ -75,12 +86,21 @@ __host__ __device__ inline int ps_tile_chains(int nb) { return nb <= 4 ? 4 : 2; 
 // Doubles per point in the per-evaluation point record (one 128-byte line).
 #define PS_REC 16
 
+// Curves of the renormalisation correction, rounded up to even (two
+// correction terms per curve; MFMA steps take four).
+__host__ __device__ inline int ps_corr_curves(int ncurves) { return (ncurves + 1) & ~1; }
+// Doubles per (chain, curve) of the per-curve sums: rows z and y of the
+// chain's matrix, [qs] each.
+__host__ __device__ inline int ps_curve_sums_width(int qs) { return 2 * qs; }
+
 // One workgroup's scratch slab in HBM (offsets in doubles, 256-byte aligned).
 struct PsSlab {
   size_t jbuf;   // [ndata+3][qs] raw Jacobian rows (data points, then the 3 convexity rows)
   size_t ybuf;   // [ndata+3] model brightness
-  size_t tsum;   // [ntiles][qs+1] per-tile column sums (column qs: brightness)
-  size_t dsum;   // [ncurves][qs+1] per-curve column sums (column qs: 1 / brightness sum)
+  size_t tsy;    // [ntiles] per-tile brightness sums
+  size_t crv;    // [ncurves] per relative curve: 1 / brightness sum
+  size_t csum;   // [chains][ncurves][ps_curve_sums_width] per-curve sums of each point chain
+  size_t ttab;   // [2 ps_corr_curves][qs] correction vectors a_k, b_k of each curve
   size_t part;   // [chains][16nb][16nb] partial augmented normal matrices
   size_t best;   // [K][K] accepted normal matrix
   size_t prec;   // [ndata][PS_REC] per-point records of the evaluated parameters (rotation angle,
@@ -100,10 +120,14 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
   o += al(rows * P.qs);
   s.ybuf = o;
   o += al(rows);
-  s.tsum = o;
-  o += al((size_t)P.ntiles * (P.qs + 1));
-  s.dsum = o;
-  o += al((size_t)P.ncurves * (P.qs + 1));
+  s.tsy = o;
+  o += al((size_t)P.ntiles);
+  s.crv = o;
+  o += al((size_t)P.ncurves);
+  s.csum = o;
+  o += al((size_t)ps_tile_chains(P.nb) * P.ncurves * ps_curve_sums_width(P.qs));
+  s.ttab = o;
+  o += al((size_t)2 * ps_corr_curves(P.ncurves) * P.qs);
   s.part = o;
   o += al((size_t)ps_tile_chains(P.nb) * 256 * P.nb * P.nb);
   s.best = o;

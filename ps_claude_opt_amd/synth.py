@@ -8,9 +8,15 @@ function of the reference's model (Start.cu:2095-2195, 2333-2403).  The
 rendering is an independent numpy forward model on a fine Fibonacci
 sphere, not the fitting code, so a scan has to rediscover the truth.
 
-Input-file grammar: period_search_BOINC.cpp:343-533.  All lightcurves are
+Input-file grammar: period_search_BOINC.cpp:343-533.  Lightcurves are
 written as relative (flag 0), the only kind the reference's GPU path
-computes.
+computes, except the last ``n_abs`` curves, which are written as calibrated
+(flag 1, absolute): their brightness is the model flux of the truth, whose
+size-scale coefficient is 0.
+
+The truth shape's Gaussian image is closed (a convex body exists) and its
+rotation is referenced to the epoch the parser derives, so the truth is a
+parameter vector of the fitted model itself (``truth_params``).
 """
 from __future__ import annotations
 
@@ -52,6 +58,7 @@ class SynthConfig:
     par_free: tuple = (1, 1, 1)
     cl_init: float = 0.1
     cl_free: int = 0
+    n_abs: int = 0                   # last n_abs curves written as calibrated (absolute)
 
 
 # the BASELINE.json workloads (configs 1, 2, 5; 3 and 4 reuse the generator)
@@ -295,8 +302,9 @@ def make_asteroid(cfg: SynthConfig, seed: int):
             coef *= 1.0 - 1e-7
         header = _header(cfg, p_start, coef, p_end)
     lines = [header, f"{len(curves)}"]
-    for cv in curves:
-        lines.append(f"{cv['t'].size} 0")
+    for ci, cv in enumerate(curves):
+        calibrated = ci >= len(curves) - cfg.n_abs
+        lines.append(f"{cv['t'].size} {1 if calibrated else 0}")
         for i in range(cv["t"].size):
             e0 = cv["sun"][i] * cv["dsun"][i]
             e = cv["earth"][i] * cv["dearth"][i]

@@ -4,8 +4,9 @@ The kernels and the oracle fix the same operation order, so every check here
 is bit-for-bit: device exp/exp2/sin/cos, facet areas, raw Jacobian rows, the
 augmented normal equations (alpha, beta, chi^2), the damped Gauss-Jordan,
 whole LM fits (dev, dark, period, lambda, beta), the convexity-weight search
-and complete period_search_out files.  Full-size configurations are checked
-on seeded samples of pairs.
+and complete period_search_out files, for relative and for mixed relative /
+calibrated (absolute) lightcurves.  Full-size configurations are checked on
+seeded samples of pairs.
 """
 import os
 
@@ -78,8 +79,7 @@ def test_jacobian_rows_bit_identical(scan, inputs, name):
     a = _perturb(O.initial_params(3, 7), O.info["ncoef"], 3)
     J0 = O.mrqcof(a, 0.0, jrows=True)[4][:N]
     J1 = scan.eval(a, 0.0, jrows=True)[4][:N]
-    assert np.array_equal(J0[:, 0], J1[:, 0], equal_nan=True)
-    assert np.array_equal(J0[:, 2:], J1[:, 2:], equal_nan=True)
+    assert np.array_equal(J0, J1, equal_nan=True)
 
 
 def test_gaussj_bit_identical(scan, inputs):
@@ -224,16 +224,60 @@ def test_error_paths(scan, tmp_path):
         assert e.value.code == -5
     finally:
         s.close()
-    # a calibrated (absolute) lightcurve is outside what the kernels support
+    # a free-parameter mask with a gap (phase-function A fixed, D and k
+    # free) is outside what the kernels support
     src = open(os.path.join(os.path.dirname(__file__), "golden", "tiny.in")).read().splitlines()
-    hdr_end = 15  # 14 header lines + curve count
-    for i in range(hdr_end, len(src)):
-        parts = src[i].split()
-        if len(parts) == 2:
-            src[i] = f"{parts[0]} 1"
-            break
-    p = tmp_path / "abs.in"
+    assert src[6].split()[1] == "1"
+    src[6] = src[6].split()[0] + " 0  phase A fixed"
+    p = tmp_path / "gap.in"
     p.write_text("\n".join(src) + "\n")
     with pytest.raises(ps.PsError) as e:
         scan.prepare_file(str(p))
     assert e.value.code == -4
+
+
+# ---- calibrated (absolute) lightcurves: no renormalisation, the size scale
+# (parameter 1) free (Start.cu:3363-3692; period_search_BOINC.cpp:343-533)
+
+@pytest.mark.parametrize("name,preset,n_abs", [("small_abs", "small", 1), ("c1_abs", "c1", 2)])
+def test_absolute_eval_bit_identical(scan, inputs, name, preset, n_abs):
+    path, _ = inputs(name, preset, None, n_abs=n_abs)
+    O = _prep(scan, path)
+    assert O.info["onlyrel"] == 0 and O.table("ia")[0] == 1
+    C = O.info["ncoef"]
+    for n, m, seed in [(1, 1, None), (3, 6, 4)]:
+        a = O.initial_params(n, m)
+        if seed is not None:
+            a = _perturb(a, C, seed)
+        A0, B0, c0, ar0, J0 = O.mrqcof(a, 0.3 ** 2, jrows=True)
+        A1, B1, c1, ar1, J1 = scan.eval(a, 0.3 ** 2, jrows=True)
+        assert np.array_equal(ar0, ar1)
+        assert np.array_equal(J0, J1)
+        assert np.array_equal(A0, A1), (name, n, m, np.abs(A0 - A1).max())
+        assert np.array_equal(B0, B1)
+        assert c0 == c1
+
+
+def test_absolute_fit_pairs_bit_identical(scan, inputs):
+    path, _ = inputs("c1_abs", "c1", None, n_abs=2)
+    O = _prep(scan, path)
+    rng = np.random.default_rng(9)
+    nl = list(rng.integers(1, O.info["n_max"] + 1, 20))
+    ml = list(rng.integers(1, 11, 20))
+    R = scan.fit_pairs(nl, ml, 0.6, 0.6)
+    _compare_pairs(O, R, nl, ml, 0.6, 0.6)
+
+
+def test_absolute_whole_application_identical(scan, inputs, tmp_path):
+    import ps_claude_opt_amd as ps
+
+    path, _ = inputs("tiny_abs", "tiny", None, n_abs=1)
+    O = po.Problem(path)
+    out_o = tmp_path / "tiny_abs_oracle.out"
+    cwo = O.run(str(out_o), threads=min(16, os.cpu_count() or 1))
+    scan.prepare_file(path)
+    cwg = scan.conw_search()
+    assert cwg == cwo
+    out_g = tmp_path / "tiny_abs_gpu.out"
+    ps.write_output(str(out_g), scan.start(cwg))
+    assert out_g.read_text() == out_o.read_text()

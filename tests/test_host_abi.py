@@ -53,7 +53,8 @@ def _arr(ptr, n):
     return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
 
 
-@pytest.mark.parametrize("name,over", [("tiny", {}), ("small", {}), ("c1", {}), ("tiny", {"lmax": 6, "nrows": 8})])
+@pytest.mark.parametrize("name,over", [("tiny", {}), ("small", {}), ("c1", {}), ("tiny", {"lmax": 6, "nrows": 8}),
+                                       ("small", {"n_abs": 1})])
 def test_product_tables_equal_oracle(inputs, name, over):
     path, _ = inputs(name + "_t" if over else name, name, None, **over)
     rc, h, err = _read_product(path)

@@ -169,6 +169,22 @@ def test_fit_from_truth_with_noise(tmp_path):
     assert abs(24 * r.period / t["period_h"] - 1) < 1e-5
 
 
+def test_fit_from_truth_absolute_curves(tmp_path):
+    """With two calibrated sparse curves the size scale is fitted too: it
+    stays at the truth's 0 up to the facet discretisation."""
+    path = str(tmp_path / "c1_abs.in")
+    t = synth.write_asteroid(path, "c1", None, {"noise": 0.0, "n_abs": 2})
+    P = po.Problem(path)
+    assert P.info["onlyrel"] == 0 and P.table("ia")[0] == 1
+    a = synth.truth_params(t)
+    cw = P.info["conw"] / P.info["escl"] ** 2
+    (r,), af = P.fit_params(a, cw, cw)
+    assert r.dev < 2e-3
+    assert abs(af[0][0]) < 0.05
+    assert abs(r.lambda_deg - t["lambda_deg"]) < 0.5
+    assert abs(r.beta_deg - t["beta_deg"]) < 0.5
+
+
 def test_scan_near_truth_finds_true_pole(tmp_path):
     """The best line of the frequencies next to the true one, 10 poles
     each from the reference's initialisation, recovers the pole (or its
