@@ -80,8 +80,10 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-configuration autotuner")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
     ap.add_argument("--warm-iters", type=int, default=20, help="LM iterations per pair of the warm-started scan")
-    ap.add_argument("--warm-refit", type=int, default=50,
-                    help="frequencies of the warm-started scan refitted with full cold fits")
+    ap.add_argument("--warm-refit", type=int, default=10,
+                    help="frequencies of the warm-started scan always refitted with full cold fits")
+    ap.add_argument("--warm-band", type=float, default=0.10,
+                    help="also refit every frequency whose truncated rms is within this fraction of the best")
     ap.add_argument("--work-dir", default=os.path.join(ROOT, "gpurun_out", "bench"))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates HBM byte counters)")
@@ -198,7 +200,8 @@ def main() -> int:
     if not os.path.exists(weights):
         weights = os.path.join(WARM_DATA, "warm_lmax6.npz")
     if not args.no_warm and n_max == info["n_max"] and os.path.exists(weights):
-        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters, args.warm_refit, weights)
+        result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters, args.warm_refit, args.warm_band,
+                                    weights)
     scan.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(path, conw_r, n_max, args.cpu_pairs)
@@ -206,11 +209,13 @@ def main() -> int:
     return 0
 
 
-def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: int, weights: str) -> dict:
+def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: int, band: float,
+               weights: str) -> dict:
     """The learned warm start on the same scan: every pair runs n_iter LM
     iterations from the network's prediction, then the `refit` best
-    frequencies are refitted cold (the plain scan's own 50-iteration fits),
-    which decide the reported winner.  The same truncated scan from the
+    frequencies and every frequency within `band` of the best truncated rms
+    are refitted cold (the plain scan's own 50-iteration fits), which decide
+    the reported winner.  The same truncated scan from the
     reference's start is timed beside it (what the network adds), and both
     winners are compared with the plain scan's (the same printed lambda and
     beta, period within 1e-5 relative, rms within 5e-4, chi2 within 0.5).
@@ -221,7 +226,7 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
     from ps_claude_opt_amd import warm
 
     ws = warm.WarmStart.load(weights)
-    out = {"n_iter": n_iter, "refit_frequencies": refit, "weights": os.path.relpath(weights, ROOT)}
+    out = {"n_iter": n_iter, "refit_min": refit, "refit_band": band, "weights": os.path.relpath(weights, ROOT)}
     modes = [("warm", ws), ("truncated_cold", None)]
     if ws.y_mean.size != scan.info["ncoef"] - 1 + 4:
         # the shipped network predicts the lmax-6 coefficients only: the
@@ -233,7 +238,7 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
     net = ws.network(scan) if len(modes) == 2 else None
     for name, w in modes:
         t0 = time.perf_counter()
-        res, refitted = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=refit)
+        res, refitted = warm.warm_scan(scan, w, net if w else None, conw_r, 1, n_max, n_iter, refit=refit, band=band)
         dt = time.perf_counter() - t0
         b = warm.best_line(res, 1, n_max, among=refitted)
         b["chi2"] = b["rms"] * b["rms"] * scan.info["ndata"]  # the output line's chi2 (start_CUDA.cu:1222-1226)
@@ -241,6 +246,7 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
                  and abs(b["period_h"] - plain_best.period_h) <= 1e-5 * plain_best.period_h
                  and abs(b["rms"] - plain_best.rms) <= 5e-4 and abs(b["chi2"] - plain_best.chi2) <= 0.5)
         out[name] = {"value": round(n_max * 10 / dt, 3), "unit": "pair-fits/s", "seconds": round(dt, 3),
+                     "refitted_frequencies": int(len(refitted)),
                      "best_line": b, "agrees_with_plain": bool(agree)}
         print(f"# {name}: {n_max * 10} pairs in {dt:.2f} s, best {b['period_h']:.6f} h rms {b['rms']:.6f} "
               f"agrees {agree}", file=sys.stderr, flush=True)

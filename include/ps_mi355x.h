@@ -257,15 +257,26 @@ int ps_mlp_free(ps_mlp* net);
  * launch shapes are compile-time constants; for callers tuning throughput
  * on their own workloads) ----
  *
- * One LM fit runs in one workgroup of waves_per_pair wavefronts (4 or 8);
- * wg_per_cu resident workgroups share a CU (0 = as many as fit).  Every
- * configuration gives bit-identical results: it changes only which
- * wavefront computes what, never the order of an operation chain.
- * Defaults: 4 and 0 (the environment variables PS_WAVES_PER_PAIR and
- * PS_WG_PER_CU override them at ps_open). */
+ * One LM fit runs in one workgroup of waves_per_pair wavefronts (1, 2, 4
+ * or 8); wg_per_cu resident workgroups share a CU (0 = as many as fit).
+ * Every configuration gives bit-identical results: it changes only which
+ * wavefront computes what, never the order of an operation chain.  A wave
+ * count whose workgroup cannot be resident for the prepared problem falls
+ * back to 4.  Defaults: 4 and 0 (the environment variables
+ * PS_WAVES_PER_PAIR and PS_WG_PER_CU override them at ps_open). */
 int ps_set_launch(ps_ctx* ctx, int waves_per_pair, int wg_per_cu);
 /* The configuration the next launch will use (after ps_prepare*). */
 int ps_get_launch(const ps_ctx* ctx, int* waves_per_pair, int* wg_per_cu);
+/* Points per tile claim of the facet pass: a wave takes 16, 64, 128 or
+ * 256 of the evaluation's view-sorted points at a time (16-point MFMA
+ * tiles, one work-counter claim per group).  Results do not depend on it.
+ * Default 16 (PS_TILE_POINTS overrides it at ps_open). */
+int ps_set_tile_points(ps_ctx* ctx, int points);
+int ps_get_tile_points(const ps_ctx* ctx, int* points);
+/* The wave counts the prepared problem can run with and each one's
+ * maximum resident workgroups per CU (up to cap entries); returns how many
+ * there are. */
+int ps_launch_options(const ps_ctx* ctx, int* waves, int* max_wg_per_cu, int cap);
 /* HIP graphs (no counterpart in the reference, which launches each kernel
  * from the host; for callers whose scan steps are short enough that launch
  * overhead shows): with on != 0, every fit launch of this context (the
@@ -276,9 +287,12 @@ int ps_get_launch(const ps_ctx* ctx, int* waves_per_pair, int* wg_per_cu);
 int ps_set_graphs(ps_ctx* ctx, int on);
 /* Graph captures and replays so far on this context (either may be NULL). */
 int ps_graph_stats(const ps_ctx* ctx, int* captures, int* replays);
-/* Times every configuration on the given work items (as ps_fit_items with
- * conw_r_dev = conw_r), keeps the fastest for later calls on this context
- * and reports it with its fit-kernel time in milliseconds. */
+/* Times the configurations on the given work items (as ps_fit_items with
+ * conw_r_dev = conw_r): every wave count x workgroups per CU at 16 points
+ * per claim, then 64, 128 and 256 points per claim at the best of those.
+ * Keeps the fastest for later calls on this context (ps_get_launch,
+ * ps_get_tile_points) and reports it with its fit-kernel time in
+ * milliseconds. */
 int ps_autotune(ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
                 const double* conw_r, int* waves_per_pair, int* wg_per_cu, double* best_ms);
 

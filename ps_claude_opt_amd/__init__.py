@@ -246,8 +246,24 @@ class Scan:
     def get_launch(self) -> dict:
         w = C.c_int(0)
         g = C.c_int(0)
+        t = C.c_int(0)
         self._check(self._lib.ps_get_launch(self._h, C.byref(w), C.byref(g)))
-        return dict(waves_per_pair=w.value, wg_per_cu=g.value)
+        self._check(self._lib.ps_get_tile_points(self._h, C.byref(t)))
+        return dict(waves_per_pair=w.value, wg_per_cu=g.value, tile_points=t.value)
+
+    def set_tile_points(self, points: int = 16):
+        """Points per facet-pass claim: 16, 64, 128 or 256."""
+        self._check(self._lib.ps_set_tile_points(self._h, points))
+
+    def launch_options(self) -> dict:
+        """{waves per pair: maximum resident workgroups per CU} of the
+        prepared problem."""
+        w = np.zeros(8, np.int32)
+        g = np.zeros(8, np.int32)
+        n = self._lib.ps_launch_options(self._h, _abi.iptr(w), _abi.iptr(g), 8)
+        if n < 0:
+            self._check(n)
+        return {int(w[i]): int(g[i]) for i in range(min(n, 8))}
 
     def set_graphs(self, on: bool = True):
         """Capture each fit launch into a HIP graph and replay it while the
@@ -272,7 +288,9 @@ class Scan:
         ms = C.c_double(0.0)
         self._check(self._lib.ps_autotune(self._h, len(p_arr), _abi.iptr(p_arr), _abi.iptr(n_arr), _abi.iptr(m_arr),
                                           _abi.dptr(cw), C.byref(w), C.byref(g), C.byref(ms)))
-        return dict(waves_per_pair=w.value, wg_per_cu=g.value, kernel_ms=ms.value)
+        t = C.c_int(0)
+        self._check(self._lib.ps_get_tile_points(self._h, C.byref(t)))
+        return dict(waves_per_pair=w.value, wg_per_cu=g.value, tile_points=t.value, kernel_ms=ms.value)
 
     def last_stats(self) -> dict:
         """Device time and work counters of the last fit call."""

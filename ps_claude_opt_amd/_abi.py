@@ -103,6 +103,9 @@ SIGNATURES = {
     "ps_mlp_free": (C.c_int, [C.c_void_p]),
     "ps_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "ps_get_launch": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "ps_set_tile_points": (C.c_int, [C.c_void_p, C.c_int]),
+    "ps_get_tile_points": (C.c_int, [C.c_void_p, _ip]),
+    "ps_launch_options": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int]),
     "ps_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ps_graph_stats": (C.c_int, [C.c_void_p, _ip, _ip]),
     "ps_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp]),

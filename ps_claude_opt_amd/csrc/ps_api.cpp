@@ -86,6 +86,33 @@ __global__ void ps_reset_counter_kernel(int* counter) {
   if (threadIdx.x == 0) *counter = 0;
 }
 
+// The compiled variants of the fit kernels, by wavefronts per workgroup.
+struct FitVariant {
+  int waves;
+  size_t (*lds_bytes)(const PsDevProblem&);
+  hipError_t (*set_attributes)(int, int, size_t);
+  int (*blocks_per_cu)(int, int, size_t);
+  hipError_t (*launch)(const PsFitArgs&, int, hipStream_t);
+  hipError_t (*phase_cycles)(unsigned long long*, bool);
+};
+const FitVariant kVariants[4] = {
+    {1, psw1::ps_fit_lds_bytes, psw1::ps_set_fit_attributes, psw1::ps_fit_blocks_per_cu, psw1::ps_launch_fit,
+     psw1::ps_phase_cycles},
+    {2, psw2::ps_fit_lds_bytes, psw2::ps_set_fit_attributes, psw2::ps_fit_blocks_per_cu, psw2::ps_launch_fit,
+     psw2::ps_phase_cycles},
+    {4, psw4::ps_fit_lds_bytes, psw4::ps_set_fit_attributes, psw4::ps_fit_blocks_per_cu, psw4::ps_launch_fit,
+     psw4::ps_phase_cycles},
+    {8, psw8::ps_fit_lds_bytes, psw8::ps_set_fit_attributes, psw8::ps_fit_blocks_per_cu, psw8::ps_launch_fit,
+     psw8::ps_phase_cycles},
+};
+int variant_index(int waves) { return waves == 1 ? 0 : waves == 2 ? 1 : waves == 8 ? 3 : 2; }
+// The variant the next launch uses: the configured one if it can be
+// resident, else the 4-wave one.
+int active_variant(const ps_ctx* c) {
+  const int v = variant_index(c->waves);
+  return c->per_cu[v] > 0 ? v : 2;
+}
+
 void drop_graph(ps_ctx* c) {
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
@@ -94,21 +121,22 @@ void drop_graph(ps_ctx* c) {
 // The scan step on the context's stream: reset of the work counter, then
 // the persistent fit launch -- eagerly, or as a captured HIP graph that is
 // replayed while the launch arguments, grid and variant stay the same.
-hipError_t launch_step(ps_ctx* c, const PsFitArgs& A, int grid, bool w8) {
+hipError_t launch_step(ps_ctx* c, const PsFitArgs& A, int grid, int v) {
+  const FitVariant& V = kVariants[v];
   if (!c->use_graphs) {
     const int zero = 0;
     hipError_t e = hipMemcpyAsync(c->counter, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) return e;
-    return w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream);
+    return V.launch(A, grid, c->stream);
   }
-  if (!c->gexec || grid != c->ggrid || w8 != c->gw8 || std::memcmp(&A, &c->gargs, sizeof(A)) != 0) {
+  if (!c->gexec || grid != c->ggrid || V.waves != c->gwaves || std::memcmp(&A, &c->gargs, sizeof(A)) != 0) {
     drop_graph(c);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ps_reset_counter_kernel, dim3(1), dim3(64), 0, c->stream, c->counter);
     hipError_t el = hipGetLastError();
-    if (el == hipSuccess) el = w8 ? psw8::ps_launch_fit(A, grid, c->stream) : psw4::ps_launch_fit(A, grid, c->stream);
+    if (el == hipSuccess) el = V.launch(A, grid, c->stream);
     e = hipStreamEndCapture(c->stream, &g);
     if (el != hipSuccess || e != hipSuccess) {
       if (g) hipGraphDestroy(g);
@@ -122,7 +150,7 @@ hipError_t launch_step(ps_ctx* c, const PsFitArgs& A, int grid, bool w8) {
     }
     std::memcpy(&c->gargs, &A, sizeof(A));
     c->ggrid = grid;
-    c->gw8 = w8;
+    c->gwaves = V.waves;
     ++c->graph_captures;
   }
   ++c->graph_replays;
@@ -182,6 +210,7 @@ PsFitArgs make_args(ps_ctx* c, int n_items) {
   A.scratch_stride = c->stride;
   A.out = c->d_out;
   A.batch = c->dps.size() > 1 ? 1 : 0;
+  A.claim_tiles = c->claim_tiles;
   return A;
 }
 
@@ -406,15 +435,15 @@ int prepare_many(ps_ctx* c, int n, const ps_problem* const* ps_, const double* c
   c->mfit = mfit;
   c->ia.assign(p0->ia, p0->ia + ma);
   const size_t lds_cap = 160 * 1024;
-  c->lds4 = psw4::ps_fit_lds_bytes(c->dps[0]);
-  c->lds8 = psw8::ps_fit_lds_bytes(c->dps[0]);
-  if (c->lds4 > lds_cap) return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
-  PS_HIP(c, psw4::ps_set_fit_attributes(nb, nbs, c->lds4));
-  c->per_cu4 = psw4::ps_fit_blocks_per_cu(nb, nbs, c->lds4);
-  if (c->per_cu4 < 1) return fail(c, PS_E_HIP, "fit kernel cannot be resident (occupancy query failed)");
-  c->per_cu8 = 0;
-  if (c->lds8 <= lds_cap && psw8::ps_set_fit_attributes(nb, nbs, c->lds8) == hipSuccess)
-    c->per_cu8 = psw8::ps_fit_blocks_per_cu(nb, nbs, c->lds8);
+  for (int v = 0; v < 4; ++v) {
+    const FitVariant& V = kVariants[v];
+    c->lds[v] = V.lds_bytes(c->dps[0]);
+    c->per_cu[v] = 0;
+    if (c->lds[v] <= lds_cap && V.set_attributes(nb, nbs, c->lds[v]) == hipSuccess)
+      c->per_cu[v] = V.blocks_per_cu(nb, nbs, c->lds[v]);
+  }
+  if (c->lds[2] > lds_cap) return fail(c, PS_E_UNSUPPORTED, "problem needs more LDS than a workgroup can have");
+  if (c->per_cu[2] < 1) return fail(c, PS_E_HIP, "fit kernel cannot be resident (occupancy query failed)");
   c->prepared = true;
   return PS_OK;
 }
@@ -451,8 +480,8 @@ int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const 
     icw[2 * k] = cw_rows[i];
     icw[2 * k + 1] = cw_dev[i];
   }
-  const bool w8 = c->waves == 8 && c->per_cu8 > 0;
-  const int per_cu = w8 ? c->per_cu8 : c->per_cu4;
+  const int v = active_variant(c);
+  const int per_cu = c->per_cu[v];
   const int wpc = c->wg_per_cu > 0 ? std::min(c->wg_per_cu, per_cu) : per_cu;
   const int grid = std::min(n_items, c->n_cu * wpc);
   int rc = ensure_batch(c, n_items, grid);
@@ -483,7 +512,7 @@ int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const 
   A.out_a = a_final ? c->d_afinal : nullptr;
   A.n_iter_fixed = n_iter_fixed;
   PS_HIP(c, hipEventRecord(c->ev0, c->stream));
-  PS_HIP(c, launch_step(c, A, grid, w8));
+  PS_HIP(c, launch_step(c, A, grid, v));
   PS_HIP(c, hipEventRecord(c->ev1, c->stream));
   std::vector<double> res((size_t)n_items * PS_OUT_STRIDE);
   PS_HIP(c, hipMemcpyAsync(res.data(), c->d_out, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -502,7 +531,7 @@ int fit_items(ps_ctx* c, int n_items, const int* prob, const int* n_list, const 
   c->last_launches = 1;
   if (std::getenv("PS_PHASE_TIMING")) {
     unsigned long long cyc[16];
-    if ((w8 ? psw8::ps_phase_cycles(cyc, true) : psw4::ps_phase_cycles(cyc, true)) == hipSuccess) {
+    if (kVariants[v].phase_cycles(cyc, true) == hipSuccess) {
       std::fprintf(stderr, "phase cycles (thread 0 of each workgroup, summed):");
       for (int i = 0; i < 10; ++i) std::fprintf(stderr, " %d:%.4g", i, (double)cyc[i]);
       std::fprintf(stderr, "\n");
@@ -614,8 +643,15 @@ int ps_open(int device, ps_ctx** out) {
     return PS_E_HIP;
   }
   c->n_cu = prop.multiProcessorCount;
-  if (const char* w = std::getenv("PS_WAVES_PER_PAIR")) c->waves = std::atoi(w) == 8 ? 8 : 4;
+  if (const char* w = std::getenv("PS_WAVES_PER_PAIR")) {
+    const int n = std::atoi(w);
+    c->waves = (n == 1 || n == 2 || n == 8) ? n : 4;
+  }
   if (const char* g = std::getenv("PS_WG_PER_CU")) c->wg_per_cu = std::max(0, std::atoi(g));
+  if (const char* t = std::getenv("PS_TILE_POINTS")) {
+    const int n = std::atoi(t);
+    c->claim_tiles = (n == 64 || n == 128 || n == 256) ? n / 16 : 1;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -979,9 +1015,23 @@ int ps_start_batch(ps_ctx* c, const int* n_start, const int* n_count, const doub
 }
 
 int ps_set_launch(ps_ctx* c, int waves_per_pair, int wg_per_cu) {
-  if (!c || (waves_per_pair != 4 && waves_per_pair != 8) || wg_per_cu < 0) return PS_E_ARG;
+  if (!c || (waves_per_pair != 1 && waves_per_pair != 2 && waves_per_pair != 4 && waves_per_pair != 8) ||
+      wg_per_cu < 0)
+    return PS_E_ARG;
   c->waves = waves_per_pair;
   c->wg_per_cu = wg_per_cu;
+  return PS_OK;
+}
+
+int ps_set_tile_points(ps_ctx* c, int points) {
+  if (!c || (points != 16 && points != 64 && points != 128 && points != 256)) return PS_E_ARG;
+  c->claim_tiles = points / 16;
+  return PS_OK;
+}
+
+int ps_get_tile_points(const ps_ctx* c, int* points) {
+  if (!c || !points) return PS_E_ARG;
+  *points = 16 * c->claim_tiles;
   return PS_OK;
 }
 
@@ -1001,11 +1051,25 @@ int ps_graph_stats(const ps_ctx* c, int* captures, int* replays) {
 
 int ps_get_launch(const ps_ctx* c, int* waves_per_pair, int* wg_per_cu) {
   if (!c || !c->prepared) return PS_E_STATE;
-  const bool w8 = c->waves == 8 && c->per_cu8 > 0;
-  const int per_cu = w8 ? c->per_cu8 : c->per_cu4;
-  if (waves_per_pair) *waves_per_pair = w8 ? 8 : 4;
+  const int v = active_variant(c);
+  const int per_cu = c->per_cu[v];
+  if (waves_per_pair) *waves_per_pair = kVariants[v].waves;
   if (wg_per_cu) *wg_per_cu = c->wg_per_cu > 0 ? std::min(c->wg_per_cu, per_cu) : per_cu;
   return PS_OK;
+}
+
+int ps_launch_options(const ps_ctx* c, int* waves, int* max_wg_per_cu, int cap) {
+  if (!c || !c->prepared) return PS_E_STATE;
+  int n = 0;
+  for (int v = 0; v < 4; ++v)
+    if (c->per_cu[v] > 0) {
+      if (n < cap) {
+        if (waves) waves[n] = kVariants[v].waves;
+        if (max_wg_per_cu) max_wg_per_cu[n] = c->per_cu[v];
+      }
+      ++n;
+    }
+  return n;
 }
 
 int ps_autotune(ps_ctx* c, int n_items, const int* prob, const int* n_list, const int* m_list, const double* conw_r,
@@ -1013,27 +1077,40 @@ int ps_autotune(ps_ctx* c, int n_items, const int* prob, const int* n_list, cons
   if (!c || n_items <= 0 || !prob || !n_list || !m_list || !conw_r) return PS_E_ARG;
   if (!c->prepared) return fail(c, PS_E_STATE, "ps_autotune before ps_prepare");
   std::vector<ps_pair_result> r(n_items);
-  int bw = 4, bg = 0;
+  int bw = 4, bg = 0, bt = 1;
   double bms = 0.0;
   bool have = false;
-  for (int w : {4, 8}) {
-    const int per_cu = w == 8 ? c->per_cu8 : c->per_cu4;
+  // waves per pair x workgroups per CU at one tile per claim, then the
+  // claim size (16 .. 256 points) at the best of those
+  for (int v = 0; v < 4; ++v) {
+    const int per_cu = c->per_cu[v];
     if (per_cu < 1) continue;
     for (int g = 1; g <= per_cu; ++g) {
-      c->waves = w;
+      c->waves = kVariants[v].waves;
       c->wg_per_cu = g;
+      c->claim_tiles = 1;
       int rc = ps_fit_items(c, n_items, prob, n_list, m_list, conw_r, conw_r, r.data());
       if (rc) return rc;
       if (!have || c->last_ms < bms) {
         have = true;
         bms = c->last_ms;
-        bw = w;
+        bw = c->waves;
         bg = g;
       }
     }
   }
   c->waves = bw;
   c->wg_per_cu = bg;
+  for (int t : {4, 8, 16}) {
+    c->claim_tiles = t;
+    int rc = ps_fit_items(c, n_items, prob, n_list, m_list, conw_r, conw_r, r.data());
+    if (rc) return rc;
+    if (c->last_ms < bms) {
+      bms = c->last_ms;
+      bt = t;
+    }
+  }
+  c->claim_tiles = bt;
   if (waves_per_pair) *waves_per_pair = bw;
   if (wg_per_cu) *wg_per_cu = bg;
   if (best_ms) *best_ms = bms;

@@ -42,12 +42,15 @@ struct ps_ctx {
   std::vector<int> ia;  // [ma] free-parameter mask shared by the problems
   std::vector<void*> dbufs;
   long long stride = 0;
-  // launch configuration: wavefronts per workgroup (one pair per workgroup)
-  // and resident workgroups per CU (0 = as many as fit)
+  // launch configuration: wavefronts per workgroup (one pair per workgroup:
+  // 1, 2, 4 or 8, compiled variants psw1 .. psw8), resident workgroups per
+  // CU (0 = as many as fit) and 16-point facet-pass tiles a wave claims at
+  // a time (1, 4, 8, 16: 16 to 256 points)
   int waves = 4;
   int wg_per_cu = 0;
-  size_t lds4 = 0, lds8 = 0;
-  int per_cu4 = 0, per_cu8 = 0;
+  int claim_tiles = 1;
+  size_t lds[4] = {0, 0, 0, 0};   // per variant (index: ps_variant_index)
+  int per_cu[4] = {0, 0, 0, 0};  // resident workgroups per CU, 0 = unusable
   // work-list buffers
   double* scratch = nullptr;
   size_t scratch_elems = 0;
@@ -67,7 +70,7 @@ struct ps_ctx {
   hipGraphExec_t gexec = nullptr;
   PsFitArgs gargs{};
   int ggrid = 0;
-  bool gw8 = false;
+  int gwaves = 0;
   int graph_captures = 0, graph_replays = 0;
   // statistics of the last fit call
   double last_ms = 0;

@@ -611,7 +611,8 @@ template <int NB>
 struct P2 {
   static constexpr int NP = NB * (NB + 1) / 2;  // lower block pairs
   static constexpr int GT = NB <= 4 ? 4 : 2;    // point chains
-  static constexpr int GB = NW / GT;            // waves per chain
+  static constexpr int GB = NW >= GT ? NW / GT : 1;   // waves per chain
+  static constexpr int CPW = NW >= GT ? 1 : GT / NW;  // chains per wave (run in turn)
   static constexpr int NPW = (NP + GB - 1) / GB;
 };
 static_assert(P2<4>::GT == 4 && P2<5>::GT == 2, "chain count must match ps_tile_chains");
@@ -662,166 +663,176 @@ template <int NB>
 __device__ __forceinline__ void phase2_chain(const PsDevProblem& P, const Slab& S, double conw_w) {
   using SH = P2<NB>;
   const int tid = tid_opaque(), lane = tid & 63, ps = lane & 15, sl = lane >> 4, w = tid >> 6;
-  const int gt = w / SH::GB, gb = w % SH::GB;
+  const int gb = w % SH::GB;
   const int mf = P.mfit, C = P.ncoef, QS = P.qs, ia1 = P.ia1;
   const int CWS = ps_curve_sums_width(QS);
   const P2Cols<NB> K2(ps, mf);
   const bool p1 = ia1 != 0 && ps == 0;  // block 0 of this lane is the size scale
   const int c0 = ps - ia1 < 0 ? 0 : ps - ia1;  // row column of block 0 (any valid one for the size scale)
   const int c1 = ps - ia1 + 16;                // row column of block 1; block b: c1 + 16 (b - 1)
-  v4d acc[SH::NPW];
+  // one chain gt of this wave
+  auto run_chain = [&](const int gt) __attribute__((always_inline)) {
+    v4d acc[SH::NPW];
 #pragma unroll
-  for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
-  struct Row {
-    double J[NB];
-    double y, sig, obs, s2;
-  };
-  // with fl: rows z and y (16 NB - 2, 16 NB - 1: block row NB - 1, r = 3,
-  // lanes sl = 2, 3) of this chain's matrix out for curve c, then cleared
-  auto flush = [&](int c, bool fl) __attribute__((always_inline)) {
-    double* o = S.csum + ((size_t)gt * P.ncurves + c) * CWS + (sl == 2 ? 0 : QS) + ps;
-#pragma unroll
-    for (int jb = 0; jb < NB; ++jb) {
-      constexpr int base = (NB - 1) * NB / 2;
-      const int idx = base + jb;
-      if (idx % SH::GB == gb) {
-        if (fl && sl >= 2) o[16 * jb] = acc[idx / SH::GB][3];
-        acc[idx / SH::GB][3] = (fl && sl >= 2) ? 0.0 : acc[idx / SH::GB][3];
-      }
-    }
-  };
-  // ALLJ: blocks 0 .. NB-2 hold Jacobian columns only (16 (NB - 1) <= mfit,
-  // true for every input the triangulation and basis produce); only the
-  // last block then needs the per-lane column kind.
-  auto tiles = [&](auto allj_tag) __attribute__((always_inline)) {
-    constexpr bool ALLJ = decltype(allj_tag)::value;
-    // first point, points in the tile, curve, curve length (negative:
-    // absolute curve), one load; past the last tile: a one-point tile at
-    // point 0
-    auto info = [&](int T) __attribute__((always_inline)) {
-      return T < P.ntiles ? *reinterpret_cast<const int4*>(P.tile_info + 4 * T) : make_int4(0, 1, 0, 1);
+    for (int i = 0; i < SH::NPW; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+    struct Row {
+      double J[NB];
+      double y, sig, obs, s2;
     };
-    auto load = [&](int4 t, int r) __attribute__((always_inline)) {
-      Row R;
-      const int pin = 4 * r + sl;
-      const int pt = t.x + (pin < t.y ? pin : 0);
-      const double* jr = S.jbuf + (size_t)pt * QS;
-      R.J[0] = jr[c0];
+    // with fl: rows z and y (16 NB - 2, 16 NB - 1: block row NB - 1, r = 3,
+    // lanes sl = 2, 3) of this chain's matrix out for curve c, then cleared
+    auto flush = [&](int c, bool fl) __attribute__((always_inline)) {
+      double* o = S.csum + ((size_t)gt * P.ncurves + c) * CWS + (sl == 2 ? 0 : QS) + ps;
 #pragma unroll
-      for (int b = 1; b < NB; ++b) R.J[b] = jr[c1 + 16 * (b - 1)];
-      R.y = S.ybuf[pt];
-      R.sig = P.sig[pt];
-      R.obs = P.bright[pt];
-      R.s2 = P.s2w[pt];
-      return R;
-    };
-    int4 ti = info(gt);
-    Row nxt = load(ti, 0);
-    for (int T = gt; T < P.ntiles; T += SH::GT) {
-      const int4 tn = info(T + SH::GT);
-      const int n_in = ti.y, c = ti.z;
-      const bool rel = ti.w > 0;
-      const double lpd = (double)(rel ? ti.w : -ti.w);
-      const double rave = rel ? S.crv[c] : 0.0;
-      // FULL: the tile has 16 points (every tile but a curve's last one);
-      // REL: a relative curve
-      auto rows = [&](auto full_tag, auto rel_tag) __attribute__((always_inline)) {
-        constexpr bool FULL = decltype(full_tag)::value;
-        constexpr bool REL = decltype(rel_tag)::value;
-        Row rw[2];
-        rw[0] = nxt;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (r < 3)
-            rw[(r + 1) & 1] = load(ti, r + 1);
-          else
-            nxt = load(tn, 0);
-          const Row& cur = rw[r & 1];
-          const bool ok = FULL || 4 * r + sl < n_in;
-          const double coef = REL ? (cur.sig * lpd) * rave : 1.0;
-          const double res = cur.obs - coef * cur.y;
-          const double s2v = ok ? cur.s2 : 0.0;
-          double X[NB], XW[NB];
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            double xr = cur.J[b];
-            if (b == 0) xr = p1 ? (REL ? 0.0 : cur.y) : xr;
-            const double xj = coef * xr;
-            double xv, xw;
-            if (ALLJ && b < NB - 1) {
-              xv = ok ? xj : 0.0;
-              xw = s2v * xv;
-            } else {
-              const bool isj = (K2.jm >> b) & 1u, isr = (K2.rm >> b) & 1u;
-              const bool isz = b == NB - 1 && K2.z, isy = b == NB - 1 && K2.y;
-              xv = isj ? xj : isr ? res : isy ? cur.y : 0.0;
-              xv = ok ? xv : 0.0;
-              xw = s2v * xv;
-              if (isz) xw = (REL && ok) ? 1.0 : 0.0;
-              if (isy) xw = REL ? s2v * cur.y : 0.0;
-            }
-            X[b] = xv;
-            XW[b] = xw;
-          }
-          p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
+      for (int jb = 0; jb < NB; ++jb) {
+        constexpr int base = (NB - 1) * NB / 2;
+        const int idx = base + jb;
+        if (idx % SH::GB == gb) {
+          if (fl && sl >= 2) o[16 * jb] = acc[idx / SH::GB][3];
+          acc[idx / SH::GB][3] = (fl && sl >= 2) ? 0.0 : acc[idx / SH::GB][3];
         }
+      }
+    };
+    // ALLJ: blocks 0 .. NB-2 hold Jacobian columns only (16 (NB - 1) <= mfit,
+    // true for every input the triangulation and basis produce); only the
+    // last block then needs the per-lane column kind.
+    auto tiles = [&](auto allj_tag) __attribute__((always_inline)) {
+      constexpr bool ALLJ = decltype(allj_tag)::value;
+      // first point, points in the tile, curve, curve length (negative:
+      // absolute curve), one load; past the last tile: a one-point tile at
+      // point 0
+      auto info = [&](int T) __attribute__((always_inline)) {
+        return T < P.ntiles ? *reinterpret_cast<const int4*>(P.tile_info + 4 * T) : make_int4(0, 1, 0, 1);
       };
-      if (rel) {
-        if (n_in == 16)
-          rows(std::true_type{}, std::true_type{});
-        else
-          rows(std::false_type{}, std::true_type{});
-        // the chain's last tile of this curve (predicated, not branched
-        // around: a branch here costs registers in the tile loop)
-        flush(c, T + SH::GT >= P.ntiles || tn.z != c);
-      } else {
-        if (n_in == 16)
-          rows(std::true_type{}, std::false_type{});
-        else
-          rows(std::false_type{}, std::false_type{});
+      auto load = [&](int4 t, int r) __attribute__((always_inline)) {
+        Row R;
+        const int pin = 4 * r + sl;
+        const int pt = t.x + (pin < t.y ? pin : 0);
+        const double* jr = S.jbuf + (size_t)pt * QS;
+        R.J[0] = jr[c0];
+#pragma unroll
+        for (int b = 1; b < NB; ++b) R.J[b] = jr[c1 + 16 * (b - 1)];
+        R.y = S.ybuf[pt];
+        R.sig = P.sig[pt];
+        R.obs = P.bright[pt];
+        R.s2 = P.s2w[pt];
+        return R;
+      };
+      int4 ti = info(gt);
+      Row nxt = load(ti, 0);
+      for (int T = gt; T < P.ntiles; T += SH::GT) {
+        const int4 tn = info(T + SH::GT);
+        const int n_in = ti.y, c = ti.z;
+        const bool rel = ti.w > 0;
+        const double lpd = (double)(rel ? ti.w : -ti.w);
+        const double rave = rel ? S.crv[c] : 0.0;
+        // FULL: the tile has 16 points (every tile but a curve's last one);
+        // REL: a relative curve
+        auto rows = [&](auto full_tag, auto rel_tag) __attribute__((always_inline)) {
+          constexpr bool FULL = decltype(full_tag)::value;
+          constexpr bool REL = decltype(rel_tag)::value;
+          Row rw[2];
+          rw[0] = nxt;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (r < 3)
+              rw[(r + 1) & 1] = load(ti, r + 1);
+            else
+              nxt = load(tn, 0);
+            const Row& cur = rw[r & 1];
+            const bool ok = FULL || 4 * r + sl < n_in;
+            const double coef = REL ? (cur.sig * lpd) * rave : 1.0;
+            const double res = cur.obs - coef * cur.y;
+            const double s2v = ok ? cur.s2 : 0.0;
+            double X[NB], XW[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              double xr = cur.J[b];
+              if (b == 0) xr = p1 ? (REL ? 0.0 : cur.y) : xr;
+              const double xj = coef * xr;
+              double xv, xw;
+              if (ALLJ && b < NB - 1) {
+                xv = ok ? xj : 0.0;
+                xw = s2v * xv;
+              } else {
+                const bool isj = (K2.jm >> b) & 1u, isr = (K2.rm >> b) & 1u;
+                const bool isz = b == NB - 1 && K2.z, isy = b == NB - 1 && K2.y;
+                xv = isj ? xj : isr ? res : isy ? cur.y : 0.0;
+                xv = ok ? xv : 0.0;
+                xw = s2v * xv;
+                if (isz) xw = (REL && ok) ? 1.0 : 0.0;
+                if (isy) xw = REL ? s2v * cur.y : 0.0;
+              }
+              X[b] = xv;
+              XW[b] = xw;
+            }
+            p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
+          }
+        };
+        if (rel) {
+          if (n_in == 16)
+            rows(std::true_type{}, std::true_type{});
+          else
+            rows(std::false_type{}, std::true_type{});
+          // the chain's last tile of this curve (predicated, not branched
+          // around: a branch here costs registers in the tile loop)
+          flush(c, T + SH::GT >= P.ntiles || tn.z != c);
+        } else {
+          if (n_in == 16)
+            rows(std::true_type{}, std::false_type{});
+          else
+            rows(std::false_type{}, std::false_type{});
+        }
+        ti = tn;
       }
-      ti = tn;
+    };
+    if (16 * (NB - 1) <= mf)
+      tiles(std::true_type{});
+    else
+      tiles(std::false_type{});
+    // the convexity pseudo-curve (rows ndata .. ndata+2, Start.cu:2539-2612):
+    // tile T = ntiles, the last tile of chain ntiles mod GT; only its first
+    // row group holds rows (lanes sl < 3); raw rows, no sums
+    if (P.ntiles % SH::GT == gt) {
+      const bool ok = sl < 3;
+      const int pt = P.ndata + (ok ? sl : 0);
+      const double* jr = S.jbuf + (size_t)pt * QS;
+      const double yv = S.ybuf[pt];
+      const double res = 0.0 - yv;
+      const double s2v = ok ? conw_w * 1.0 : 0.0;
+      double X[NB], XW[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int par = 16 * b + ps + 2 - ia1;  // parameter of the lane's column (1-based)
+        const double jv = b == 0 ? jr[c0] : jr[c1 + 16 * (b - 1)];
+        double xv = (K2.jm >> b) & 1u ? (par == 1 ? yv : par <= C ? jv : 0.0) : (K2.rm >> b) & 1u ? res : 0.0;
+        xv = ok ? xv : 0.0;
+        X[b] = xv;
+        XW[b] = s2v * xv;
+      }
+      p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
     }
+    double* pt = S.part + (size_t)gt * 256 * NB * NB;
+    const int ld = 16 * NB;
+    int idx = 0;
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int jb = 0; jb <= ib; ++jb) {
+        if (idx % SH::GB == gb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) pt[(size_t)(16 * ib + 4 * r + sl) * ld + 16 * jb + ps] = acc[idx / SH::GB][r];
+        }
+        ++idx;
+      }
   };
-  if (16 * (NB - 1) <= mf)
-    tiles(std::true_type{});
-  else
-    tiles(std::false_type{});
-  // the convexity pseudo-curve (rows ndata .. ndata+2, Start.cu:2539-2612):
-  // tile T = ntiles, the last tile of chain ntiles mod GT; only its first
-  // row group holds rows (lanes sl < 3); raw rows, no sums
-  if (P.ntiles % SH::GT == gt) {
-    const bool ok = sl < 3;
-    const int pt = P.ndata + (ok ? sl : 0);
-    const double* jr = S.jbuf + (size_t)pt * QS;
-    const double yv = S.ybuf[pt];
-    const double res = 0.0 - yv;
-    const double s2v = ok ? conw_w * 1.0 : 0.0;
-    double X[NB], XW[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int par = 16 * b + ps + 2 - ia1;  // parameter of the lane's column (1-based)
-      const double jv = b == 0 ? jr[c0] : jr[c1 + 16 * (b - 1)];
-      double xv = (K2.jm >> b) & 1u ? (par == 1 ? yv : par <= C ? jv : 0.0) : (K2.rm >> b) & 1u ? res : 0.0;
-      xv = ok ? xv : 0.0;
-      X[b] = xv;
-      XW[b] = s2v * xv;
-    }
-    p2_products<NB, SH::NPW, SH::GB>(X, XW, gb, acc);
+  // the chains of this wave: one (NW >= GT), or GT / NW in turn
+  if constexpr (SH::CPW == 1) {
+    run_chain(w / SH::GB);
+  } else {
+#pragma unroll 1
+    for (int cw = 0; cw < SH::CPW; ++cw) run_chain(w + cw * NW);
   }
-  double* pt = S.part + (size_t)gt * 256 * NB * NB;
-  const int ld = 16 * NB;
-  int idx = 0;
-#pragma unroll
-  for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-    for (int jb = 0; jb <= ib; ++jb) {
-      if (idx % SH::GB == gb) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) pt[(size_t)(16 * ib + 4 * r + sl) * ld + 16 * jb + ps] = acc[idx / SH::GB][r];
-      }
-      ++idx;
-    }
 }
 
 // Brightness sums of the data tiles (16 consecutive points of one curve, in
@@ -932,7 +943,7 @@ __device__ __forceinline__ void combine_correct(const PsDevProblem& P, const Sla
 // raw Jacobian rows of every point (and the convexity rows) in the slab.
 template <int NB, int NBS>
 __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, const double* a, double conw_r2, const Slab& S,
-                         unsigned long long* vis_count) {
+                         unsigned long long* vis_count, int claim) {
   using SH = P2<NB>;
   const int tid = tid_opaque(), lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, in a scalar register
@@ -1058,17 +1069,19 @@ __device__ __forceinline__ void evaluate(const PsDevProblem& P, const Lds& L, co
   __builtin_amdgcn_s_setprio(0);
   PS_TMARK(9);
   // phase 1 over the points in view-direction order, 16 per tile; each wave
-  // takes the next unclaimed tile (tiles differ in cost by their visible
-  // facet steps; the assignment changes no result)
+  // takes the next `claim` unclaimed tiles (tiles differ in cost by their
+  // visible facet steps; the assignment changes no result)
   double* rowbuf = L.rowbuf + w * 256;
   unsigned long long nvis = 0;
   const int n_tiles = (N + 15) >> 4;
+  claim = claim < 1 ? 1 : claim;  // every claim takes at least one tile: the loop ends
   for (;;) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(next_tile, 1);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if (t >= n_tiles) break;
-    nvis += phase1_tile<NB, NBS>(P, L, S.prec, S.perm, t, S.jbuf, S.ybuf, rowbuf);
+    int t0 = 0;
+    if (lane == 0) t0 = atomicAdd(next_tile, claim);
+    t0 = __builtin_amdgcn_readfirstlane(t0);
+    if (t0 >= n_tiles) break;
+    const int t1 = t0 + claim < n_tiles ? t0 + claim : n_tiles;
+    for (int t = t0; t < t1; ++t) nvis += phase1_tile<NB, NBS>(P, L, S.prec, S.perm, t, S.jbuf, S.ybuf, rowbuf);
   }
   PS_TMARK(1);
   __syncthreads();
@@ -1384,7 +1397,7 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
         __syncthreads();
         PS_TMARK(7);
       }
-      evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, conw_r2_rows, SL, &S.nvis);
+      evaluate<NB, NBS>(P, L, pass == 0 ? L.cgg : L.atry, conw_r2_rows, SL, &S.nvis, A.claim_tiles);
       if (pass == 0) {
         for (int e = tid; e < K * K; e += WG) best[e] = L.work[e];
         if (tid == 0) S.ochisq = L.work[mf * K + mf];
@@ -1490,7 +1503,7 @@ constexpr int fit_waves_per_eu() {
 #ifdef PS_FIT_WAVES_PER_EU
   return PS_FIT_WAVES_PER_EU;
 #else
-  return NW == 4 && NB <= 4 ? 3 : 2;
+  return NW == 4 && NB <= 4 ? 3 : 2;  // 1 and 2 waves per pair: LDS limits the workgroups per CU
 #endif
 }
 
@@ -1531,7 +1544,7 @@ __global__ __launch_bounds__(WG) void ps_eval_kernel(PsFitArgs A, const double* 
   const Slab SL = make_slab(P, A.scratch);
   for (int i = tid; i < P.ma; i += WG) L.cgg[i] = a[i];
   __syncthreads();
-  evaluate<NB, NBS>(P, L, L.cgg, A.conw_r2_rows, SL, nullptr);
+  evaluate<NB, NBS>(P, L, L.cgg, A.conw_r2_rows, SL, nullptr, 1);
   for (int e = tid; e < K * K; e += WG) out_work[e] = L.work[e];
   for (int f = tid; f < P.nfac; f += WG) out_area[f] = L.nrm[4 * f + 3];
   if (out_j != nullptr) {

@@ -66,6 +66,7 @@ struct PsFitArgs {
   double* out_a;        // [n_items][ma] final parameters, or null
   int n_iter_fixed;     // > 0: exactly this many LM iterations (the reference's stopCondition > 1 mode)
                         // instead of each problem's own stop condition
+  int claim_tiles;      // 16-point facet-pass tiles a wave claims at a time (1, 4, 8, 16)
   double conw_r2_rows;  // ps_eval only: sigr2 of the convexity rows
   int batch;            // 1: items of several problems (read from probs), 0: every item is on P0
 };
@@ -143,7 +144,7 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
 }
 
 // Launch API of one compiled variant of the fit kernels (namespace = the
-// number of wavefronts per workgroup: psw4, psw8).
+// number of wavefronts per workgroup: psw1, psw2, psw4, psw8).
 #define PS_DECLARE_FIT_VARIANT(NS)                                                                  \
   namespace NS {                                                                                    \
   size_t ps_fit_lds_bytes(const PsDevProblem& P);                                                   \
@@ -158,5 +159,7 @@ __host__ __device__ inline PsSlab ps_slab(const PsDevProblem& P) {
   hipError_t ps_launch_math(int n, const double* x, double* out, double* out2, hipStream_t s);      \
   }
 
+PS_DECLARE_FIT_VARIANT(psw1)
+PS_DECLARE_FIT_VARIANT(psw2)
 PS_DECLARE_FIT_VARIANT(psw4)
 PS_DECLARE_FIT_VARIANT(psw8)

@@ -185,20 +185,23 @@ def close_gaussian_image(Y: np.ndarray, nrm: np.ndarray, coef: np.ndarray, lmax:
     c[[iz, ix, iy]] = 0.0
     h = Y @ c
     h -= h.max()
+    # the basis's own l = 1 columns (the normal's components up to rounding),
+    # so the solved coefficients close the areas exp(Y @ c) exactly as used
+    Y1 = Y[:, [ix, iy, iz]]
     g = np.zeros(3)
     w = np.exp(h)
     for _ in range(100):
         grad = nrm.T @ w
         if np.linalg.norm(grad) <= 1e-15 * w.sum():
             break
-        hess = (nrm * w[:, None]).T @ nrm
+        hess = (nrm * w[:, None]).T @ Y1
         step = np.linalg.solve(hess, grad)
-        # damped Newton: halve the step until Phi decreases
-        phi = w.sum()
+        # damped Newton: halve the step until |sum A n| decreases
+        gn = np.linalg.norm(grad)
         t = 1.0
         while True:
-            w_new = np.exp(h + nrm @ (g - t * step))
-            if w_new.sum() <= phi or t < 1e-12:
+            w_new = np.exp(h + Y1 @ (g - t * step))
+            if np.linalg.norm(nrm.T @ w_new) < gn or t < 1e-12:
                 break
             t *= 0.5
         g = g - t * step

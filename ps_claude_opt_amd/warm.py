@@ -278,13 +278,32 @@ def best_lines(res: np.ndarray, n_freq: int):
     return best
 
 
+REFIT_MIN = 10      # frequencies always refitted cold
+REFIT_BAND = 0.10   # and every frequency whose truncated rms is within 10 % of the best one
+
+
+def refit_set(rms: np.ndarray, refit: int = REFIT_MIN, band: float = REFIT_BAND) -> np.ndarray:
+    """Frequencies (0-based, sorted) a warm scan refits cold: the `refit`
+    lowest truncated rms, and every frequency whose truncated rms is at most
+    (1 + band) times the lowest.  The band makes the set adaptive: where
+    the truncated screen separates its best frequencies clearly, few are
+    refitted; where many lie close to the best, all of those are."""
+    r = np.where(np.isnan(rms), np.inf, rms)
+    order = np.argsort(r, kind="stable")
+    k = min(refit, r.size)
+    if r.size and np.isfinite(r[order[0]]):
+        k = max(k, int(np.searchsorted(r[order], r[order[0]] * (1.0 + band), side="right")))
+    return np.sort(order[:k])
+
+
 def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, n_start: int, n_count: int,
-              n_iter: int, refit: int = 10):
+              n_iter: int, refit: int = REFIT_MIN, band: float = REFIT_BAND):
     """Warm-started scan of problem 0: every pair runs n_iter LM iterations
     from the network's prediction (ws None: from the reference's start);
-    then the `refit` frequencies with the lowest rms are fitted again cold,
-    with the problem's own stop condition (all 10 poles), and those cold
-    results replace the truncated ones.  Returns (results [n_count*10]
+    then the frequencies of refit_set (the `refit` lowest truncated rms and
+    all within `band` of the lowest) are fitted again cold, with the
+    problem's own stop condition (all 10 poles), and those cold results
+    replace the truncated ones.  Returns (results [n_count*10]
     frequency-major, refitted frequency indices).  The scan's winner is the
     best of the refitted frequencies (best_line(..., among=refitted)):
     those are plain fits, and a warm-started truncated fit may end below a
@@ -296,8 +315,7 @@ def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, 
     res, _ = scan.fit_items_from(p, n, m, conw_r, conw_r, a0, n_iter=n_iter)
     best = best_lines(res, n_count)
     rms = res["dev"].reshape(n_count, 10)[np.arange(n_count), best]
-    order = np.argsort(np.where(np.isnan(rms), np.inf, rms), kind="stable")[:refit]
-    fr = np.sort(order)
+    fr = refit_set(rms, refit, band)
     if fr.size:
         n2 = np.repeat(n_start + fr, 10)
         m2 = np.tile(np.arange(1, 11), fr.size)

@@ -50,8 +50,14 @@ def _one_thread():
 def generate(directory: str, first: int, count: int, workers: int, preset: str = "c2"):
     os.makedirs(directory, exist_ok=True)
     jobs = [(os.path.join(directory, f"a{first + i:05d}.in"), first + i, preset) for i in range(count)]
+    out = []
+    t0 = time.time()
     with Pool(workers, initializer=_one_thread) as pool:
-        return pool.map(_write_one, jobs)
+        for r in pool.imap(_write_one, jobs, chunksize=16):
+            out.append(r)
+            if len(out) % 1000 == 0:
+                print(f"# {len(out)} of {count} asteroids written, {time.time() - t0:.0f} s", flush=True)
+    return out
 
 
 def true_freq_index(scan, periods_h) -> np.ndarray:

@@ -93,8 +93,9 @@ def test_batch_rejects_different_basis(batch_files, tmp_path):
 
 
 def test_launch_configurations_bit_identical(inputs):
-    """4 or 8 wavefronts per pair, 1 or 2 workgroups per CU: the same
-    results bit for bit (only which wavefront computes what changes)."""
+    """1, 2, 4 or 8 wavefronts per pair, 1 or all resident workgroups per
+    CU, 16 to 256 points per facet-pass claim: the same results bit for bit
+    (only which wavefront computes what changes)."""
     import ps_claude_opt_amd as ps
 
     path, _ = inputs("c1")
@@ -103,17 +104,83 @@ def test_launch_configurations_bit_identical(inputs):
     ml = rng.integers(1, 11, 48)
     with ps.Scan(0) as s:
         s.prepare_file(path)
+        opts = s.launch_options()
+        assert set(opts) == {1, 2, 4, 8}, opts
         ref = None
-        for w, g in [(4, 0), (4, 1), (8, 0), (8, 1)]:
+        for w, g, t in [(4, 0, 16), (4, 1, 16), (8, 0, 16), (8, 1, 16), (2, 0, 16), (1, 0, 16), (4, 0, 64),
+                        (4, 0, 128), (2, 0, 256)]:
             s.set_launch(w, g)
+            s.set_tile_points(t)
             got = s.get_launch()
-            assert got["waves_per_pair"] == w
+            assert got["waves_per_pair"] == w and got["tile_points"] == t
             R = s.fit_pairs(nl, ml, 0.6)
             if ref is None:
                 ref = R
             else:
                 for f in FIELDS:
-                    assert all(_same(a, b) for a, b in zip(ref[f], R[f])), (w, g, f)
+                    assert all(_same(a, b) for a, b in zip(ref[f], R[f])), (w, g, t, f)
         best = s.autotune(np.zeros(48, np.int32), nl, ml, 0.6)
-        assert best["waves_per_pair"] in (4, 8) and best["wg_per_cu"] >= 1 and best["kernel_ms"] > 0
-        assert s.get_launch() == {"waves_per_pair": best["waves_per_pair"], "wg_per_cu": best["wg_per_cu"]}
+        assert best["waves_per_pair"] in opts and 1 <= best["wg_per_cu"] <= opts[best["waves_per_pair"]]
+        assert best["tile_points"] in (16, 64, 128, 256) and best["kernel_ms"] > 0
+        got = s.get_launch()
+        assert got == {k: best[k] for k in ("waves_per_pair", "wg_per_cu", "tile_points")}
+
+
+@pytest.fixture(scope="module")
+def config3_files(tmp_path_factory):
+    """The batched ragged workload: 64 seeded lmax-6 asteroids with 300 to
+    8,000 points (dense : sparse = 1 : 2) and 1,000-4,000 frequencies each."""
+    from ps_claude_opt_amd import synth
+
+    d = str(tmp_path_factory.mktemp("c3"))
+    truths = synth.write_batch(d, 64)
+    return [t["path"] for t in truths], truths
+
+
+def test_config3_workload_sample(config3_files):
+    """One persistent launch over all 64 asteroids of the ragged workload:
+    a seeded sample of 128 items (two per asteroid) is bit-identical to the
+    CPU oracle and, for eight asteroids, to the same pairs fitted alone; the
+    autotuner then runs on the batch and its choice gives the same results."""
+    import ps_claude_opt_amd as ps
+
+    paths, truths = config3_files
+    npts = [t["n_points"] for t in truths]
+    assert min(npts) >= 300 and max(npts) <= 8200 and max(npts) > 4000
+    rng = np.random.default_rng(33)
+    with ps.Scan(0) as b:
+        b.prepare_files(paths)
+        info = b.batch_info
+        assert len(info) == 64 and all(1000 <= x["n_max"] <= 4000 for x in info)
+        prob = np.repeat(np.arange(64), 2)
+        nl = np.array([rng.integers(1, info[p]["n_max"] + 1) for p in prob])
+        ml = rng.integers(1, 11, prob.size)
+        cw = 0.5 + 0.01 * prob
+        R = b.fit_items(prob, nl, ml, cw, cw)
+        best = b.autotune(prob, nl, ml, cw)
+        assert best["kernel_ms"] > 0
+        R2 = b.fit_items(prob, nl, ml, cw, cw)
+    for f in FIELDS:
+        assert all(_same(x, y) for x, y in zip(R[f], R2[f])), f
+    from concurrent.futures import ThreadPoolExecutor
+
+    def oracle(p):
+        sel = np.nonzero(prob == p)[0]
+        O = po.Problem(paths[p])
+        ores, _ = O.fit_pairs(nl[sel], ml[sel], float(cw[sel[0]]), float(cw[sel[0]]), threads=2)
+        return p, sel, ores
+
+    # the oracle's ctypes calls release the GIL: asteroids run side by side
+    with ThreadPoolExecutor(max_workers=max(1, THREADS // 2)) as ex:
+        for p, sel, ores in ex.map(oracle, range(64)):
+            for k, i in enumerate(sel):
+                for f in FIELDS:
+                    assert _same(R[i][f], float(getattr(ores[k], f))), (p, i, f)
+    for p in range(0, 64, 8):
+        sel = np.nonzero(prob == p)[0]
+        with ps.Scan(0) as s:
+            s.prepare_file(paths[p])
+            S = s.fit_pairs(nl[sel], ml[sel], float(cw[sel[0]]), float(cw[sel[0]]))
+        for k, i in enumerate(sel):
+            for f in FIELDS:
+                assert _same(R[i][f], S[k][f]), (p, i, f)

@@ -69,6 +69,36 @@ def test_bf16_gradients_match_fp64(scan):
     net.close()
 
 
+def test_shipped_network_shape_forward_and_gradients(scan):
+    """The shipped lmax-6 network's shape, 528 -> 256 x 3 -> 52, with its
+    trained parameters: forward in FP32 and BF16 and the BF16 gradients
+    against the FP64 restatement, on standardised features of real
+    (frequency, pole) items."""
+    from ps_claude_opt_amd import warm
+
+    ws = warm.WarmStart.load(warm.DEFAULT_WEIGHTS)
+    assert (ws.n_in, ws.n_hidden, ws.n_layers, ws.n_out) == (528, 256, 3, 52)
+    net = ws.network(scan)
+    ref = mlp_ref.Mlp(528, 256, 3, 52)
+    p = net.get_params().astype(np.float64)
+    assert np.array_equal(net.get_params(), ws.params)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(96, 528)).astype(np.float32)
+    y_ref, _, _ = ref.forward(p, x.astype(np.float64))
+    assert _rel(net.forward(x, bf16=False), y_ref) < 1e-5
+    assert _rel(net.forward(x, bf16=True), y_ref) < 3e-2
+    t = rng.normal(size=(96, 52)).astype(np.float32)
+    loss, g = net.loss_grad(x, t)
+    l_ref, g_ref = ref.loss_grad(p, x.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - l_ref) < 2e-2 * l_ref
+    o = 0
+    for k, n in zip(ref.dims[:-1], ref.dims[1:]):
+        for size in (k * n, n):
+            assert _rel(g[o:o + size], g_ref[o:o + size]) < 5e-2, (k, n, size)
+            o += size
+    net.close()
+
+
 def test_first_adam_step_and_training_reduce_loss(scan):
     net = _net(scan, seed=5)
     ref = mlp_ref.Mlp(40, 32, 3, 12)
@@ -251,6 +281,33 @@ def test_warm_scan_refits_are_plain_fits(inputs):
         assert w_best == p_best
     else:
         assert w_best["rms"] >= p_best["rms"]
+
+
+def test_warm_scan_config2_first_500_frequencies(inputs):
+    """The warm scan on a config-2 workload (the c2 preset's asteroid, its
+    first 500 of 5,000 frequencies, 10 poles each) with the shipped lmax-6
+    network: the refitted frequencies carry the plain fits bit for bit, and
+    the scan reports the plain winner."""
+    import ps_claude_opt_amd as ps
+    from ps_claude_opt_amd import warm
+
+    path, _ = inputs("c2")
+    with ps.Scan(0) as s:
+        s.prepare_file(path)
+        assert s.info["ndata"] == 2700 and s.info["n_max"] == 5000
+        cw = s.conw_search()
+        n_max = 500
+        n = np.repeat(np.arange(1, n_max + 1), 10)
+        m = np.tile(np.arange(1, 11), n_max)
+        plain, _ = s.fit_items_from(np.zeros(n.size, np.int32), n, m, cw, cw, None)
+        ws = warm.WarmStart.load(warm.DEFAULT_WEIGHTS)
+        net = ws.network(s)
+        res, fr = warm.warm_scan(s, ws, net, cw, 1, n_max, n_iter=20)
+        net.close()
+    rows = ((fr - 1)[:, None] * 10 + np.arange(10)).ravel()
+    for f in ("dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "niter"):
+        assert np.array_equal(res[f][rows], plain[f][rows], equal_nan=True), f
+    assert warm.best_line(res, 1, n_max, among=fr) == warm.best_line(plain, 1, n_max)
 
 
 def test_lmax8_network_warm_fits_bit_identical_to_oracle(inputs):

@@ -30,6 +30,11 @@ def test_library_exports_every_declared_symbol():
     for n in sorted(names):
         assert hasattr(lib, n), n
     assert names == set(_abi.SIGNATURES), names ^ set(_abi.SIGNATURES)
+    # the command line's BOINC shim, in the same library
+    boinc = set(re.findall(r"\b(ps_boinc_[a-z0-9_]+)\s*\(", open(os.path.join(ROOT, "include", "ps_boinc.h")).read()))
+    assert len(boinc) == 7
+    for n in sorted(boinc):
+        assert hasattr(lib, n), n
 
 
 def _read_product(path):
