@@ -27,7 +27,8 @@
 // --max-batches K stops after K scan batches, leaving a resumable state
 // (for time-sliced runs).
 // Exit status: 0 done, 1 output not writable, 2 bad input, 3 device
-// failure during the scan, 999 no usable device.
+// failure during the scan, 999 no usable device (as start_CUDA.cu:216;
+// the parent process reads it modulo 256, as 231).
 #include <sys/stat.h>
 #include <unistd.h>
 
