@@ -190,6 +190,30 @@ int ps_conw_search_batch(ps_ctx* ctx, double* conw_r /* [n] */);
  * i with conw_r[i]; the lines of problem i follow those of problems < i. */
 int ps_start_batch(ps_ctx* ctx, const int* n_start, const int* n_count, const double* conw_r, ps_line* out);
 
+/* ---- allocation-free fits (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc) ----
+ *
+ * ps_fit_items on caller-owned DEVICE buffers, allocating nothing: item k is
+ * pair (d_n[k], d_m[k]) (1-based) of problem d_prob[k] (d_prob may be NULL
+ * when one problem is prepared), with d_cw[2k] the SQUARE of the convexity
+ * rows' weight (conw_r[k]^2 for ps_fit_items' conw_r, the factor the rows'
+ * squared residuals carry) and d_cw[2k+1] the conwR of Iter2's dev
+ * (ps_fit_items' conw_r_dev[k], not squared).  Results go to
+ * d_out[k*10 ..]: dev, dark, period [days], lambda, beta, chisq, LM
+ * iterations, reported (1), visible (point, facet) pairs, unused.
+ *
+ * The work counter and the workgroups' scratch slabs live in a file-scope
+ * device array of 256 MiB (each GPU the library runs on holds one for the
+ * life of the process): the launch uses at most as many workgroups as slabs
+ * fit, and a problem whose single slab exceeds it is rejected with
+ * PS_E_ARG.  Every call on a GPU shares that GPU's array and leaves its
+ * contents behind (the counter is cleared by a kernel at each call): calls
+ * that overlap on one GPU race.  Synchronous on the context's stream;
+ * results are bit-identical to ps_fit_items. */
+int ps_fit_items_static(ps_ctx* ctx, int n_items, const int* d_prob, const int* d_n, const int* d_m,
+                        const double* d_cw, double* d_out);
+
 /* ---- learned warm start (no counterpart in the reference; for callers
  * that start each LM fit from a predicted shape instead of the ellipsoid of
  * ellfit.c) ----

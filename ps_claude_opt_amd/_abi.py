@@ -103,6 +103,8 @@ SIGNATURES = {
     "ps_mlp_free": (C.c_int, [C.c_void_p]),
     "ps_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "ps_get_launch": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "ps_fit_items_static": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     "ps_set_tile_points": (C.c_int, [C.c_void_p, C.c_int]),
     "ps_get_tile_points": (C.c_int, [C.c_void_p, _ip]),
     "ps_launch_options": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int]),

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,20 @@ constexpr int kMaxIter = 60;  // constants.h:12
 }
 
 #include "ps_ctx.h"
+
+// Device memory of ps_fit_items_static, the entry point for callers that
+// cannot allocate: the work-queue counter (its first 256 bytes) and the
+// workgroups' scratch slabs.  Every GPU the library runs on holds one copy
+// for the life of the process.  In host code `scratch` names a host-side
+// stand-in, not device memory: only the address that hipGetSymbolAddress
+// returns for the current device (kept per device at ps_open) may be used.
+static __device__ __align__(256) double scratch[PS_STATIC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double* g_static_scratch[kMaxDevices] = {};  // scratch's address on each device (host table)
+std::mutex g_static_scratch_mu;               // guards the table (contexts may open concurrently)
+}
 
 namespace {
 
@@ -643,6 +658,19 @@ int ps_open(int device, ps_ctx** out) {
     return PS_E_HIP;
   }
   c->n_cu = prop.multiProcessorCount;
+  // the static scratch's address on this device, once per device
+  if (device < kMaxDevices) {
+    std::lock_guard<std::mutex> lock(g_static_scratch_mu);
+    if (!g_static_scratch[device]) {
+      void* sp = nullptr;
+      if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) {
+        delete c;
+        return PS_E_HIP;
+      }
+      g_static_scratch[device] = static_cast<double*>(sp);
+    }
+    c->static_scratch = g_static_scratch[device];
+  }
   if (const char* w = std::getenv("PS_WAVES_PER_PAIR")) {
     const int n = std::atoi(w);
     c->waves = (n == 1 || n == 2 || n == 8) ? n : 4;
@@ -882,6 +910,50 @@ int ps_warm_features(const ps_ctx* c, int n_items, const int* prob, const int* n
     f[base + 9] = (float)S.mphase;
     f[base + 10] = (float)std::log(period_h / 10.0);
   }
+  return PS_OK;
+}
+
+int ps_fit_items_static(ps_ctx* c, int n_items, const int* d_prob, const int* d_n, const int* d_m,
+                        const double* d_cw, double* d_out) {
+  if (!c) return PS_E_ARG;
+  if (!c->prepared) return fail(c, PS_E_STATE, "ps_fit_items_static before ps_prepare");
+  if (n_items <= 0 || !d_n || !d_m || !d_cw || !d_out || (c->dps.size() > 1 && !d_prob))
+    return fail(c, PS_E_ARG, "ps_fit_items_static: bad item list");
+  if (!c->static_scratch) return fail(c, PS_E_HIP, "no static scratch on this device");
+  // the counter in the first 256 bytes, then as many slabs as fit
+  constexpr size_t head = 256 / sizeof(double), total = PS_STATIC_SCRATCH_BYTES / sizeof(double);
+  const long long max_slabs = (long long)((total - head) / (size_t)c->stride);
+  if (max_slabs < 1)
+    return fail(c, PS_E_ARG, "ps_fit_items_static: one scratch slab of this problem exceeds the static scratch");
+  PS_HIP(c, hipSetDevice(c->device));
+  const int v = active_variant(c);
+  const int per_cu = c->per_cu[v];
+  const int wpc = c->wg_per_cu > 0 ? std::min(c->wg_per_cu, per_cu) : per_cu;
+  const int grid = (int)std::min<long long>({(long long)n_items, (long long)c->n_cu * wpc, max_slabs});
+  // the allocating path's arguments with the caller's buffers and the
+  // static array in place of the context's
+  PsFitArgs A = make_args(c, n_items);
+  A.item_p = d_prob;
+  A.item_n = d_n;
+  A.item_m = d_m;
+  A.item_cw = d_cw;
+  A.counter = reinterpret_cast<int*>(c->static_scratch);
+  A.scratch = c->static_scratch + head;
+  A.out = d_out;
+  // the counter is cleared by a kernel (no memset node if the caller
+  // captures the stream), then the persistent launch
+  PS_HIP(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(ps_reset_counter_kernel, dim3(1), dim3(64), 0, c->stream, A.counter);
+  PS_HIP(c, hipGetLastError());
+  PS_HIP(c, kVariants[v].launch(A, grid, c->stream));
+  PS_HIP(c, hipEventRecord(c->ev1, c->stream));
+  PS_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  PS_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  c->last_launches = 1;
+  c->last_pairs = n_items;
+  c->last_vis = c->last_iters = c->last_evals = 0;
   return PS_OK;
 }
 

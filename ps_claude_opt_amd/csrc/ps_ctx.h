@@ -51,6 +51,7 @@ struct ps_ctx {
   int claim_tiles = 1;
   size_t lds[4] = {0, 0, 0, 0};   // per variant (index: ps_variant_index)
   int per_cu[4] = {0, 0, 0, 0};  // resident workgroups per CU, 0 = unusable
+  double* static_scratch = nullptr;  // this device's address of the file-scope scratch array
   // work-list buffers
   double* scratch = nullptr;
   size_t scratch_elems = 0;

@@ -6,6 +6,7 @@
 #include <cstddef>
 
 #define PS_OUT_STRIDE 10   // doubles per pair result
+#define PS_STATIC_SCRATCH_BYTES ((size_t)256 << 20)  // device memory of ps_fit_items_static
 #define PS_FACET_PAD 16    // zero facets after the last one in the basis table and the LDS normals
 
 // Device pointers and scalars of one prepared problem (passed by value).
