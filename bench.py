@@ -79,6 +79,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-configuration autotuner")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
+    ap.add_argument("--no-screen", action="store_true", help="skip the learned frequency screen's scan")
     ap.add_argument("--warm-iters", type=int, default=20, help="LM iterations per pair of the warm-started scan")
     ap.add_argument("--warm-refit", type=int, default=10,
                     help="frequencies of the warm-started scan always refitted with full cold fits")
@@ -201,7 +202,7 @@ def main() -> int:
         weights = os.path.join(WARM_DATA, "warm_lmax6.npz")
     if not args.no_warm and n_max == info["n_max"] and os.path.exists(weights):
         result["warm"] = bench_warm(scan, conw_r, n_max, best, args.warm_iters, args.warm_refit, args.warm_band,
-                                    weights)
+                                    weights, screen_ok=args.preset == "c2" and not args.no_screen)
     scan.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(path, conw_r, n_max, args.cpu_pairs)
@@ -210,7 +211,7 @@ def main() -> int:
 
 
 def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: int, band: float,
-               weights: str) -> dict:
+               weights: str, screen_ok: bool = False) -> dict:
     """The learned warm start on the same scan: every pair runs n_iter LM
     iterations from the network's prediction, then the `refit` best
     frequencies and every frequency within `band` of the best truncated rms
@@ -220,7 +221,9 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
     winners are compared with the plain scan's (the same printed lambda and
     beta, period within 1e-5 relative, rms within 5e-4, chi2 within 0.5).
     How often each finds the plain winner on held-out asteroids is measured
-    by ps_claude_opt_amd/warm_eval.py."""
+    by ps_claude_opt_amd/warm_eval.py.  With screen_ok, the learned frequency
+    screen's scan is timed as well: only its kept frequencies get truncated
+    fits (config-2-style asteroids, the ones it was trained on)."""
     import numpy as np
 
     from ps_claude_opt_amd import warm
@@ -252,6 +255,30 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
               f"agrees {agree}", file=sys.stderr, flush=True)
     if net is not None:
         net.close()
+    if screen_ok:
+        # the learned frequency screen: fold features and the screen network
+        # for every frequency, truncated cold fits of the kept ones, cold
+        # refits as above (ps_claude_opt_amd/screen.py)
+        from ps_claude_opt_amd import screen
+
+        sc = screen.Screen.load()
+        t0 = time.perf_counter()
+        snet = sc.network(scan)
+        res, refitted, kept = screen.screened_scan(scan, sc, snet, conw_r, 1, n_max, n_iter, refit=refit, band=band)
+        snet.close()
+        dt = time.perf_counter() - t0
+        b = warm.best_line(res, 1, n_max, among=refitted)
+        b["chi2"] = b["rms"] * b["rms"] * scan.info["ndata"]
+        agree = (round(b["lambda_deg"]) == round(plain_best.lambda_deg) and round(b["beta_deg"]) == round(plain_best.beta_deg)
+                 and abs(b["period_h"] - plain_best.period_h) <= 1e-5 * plain_best.period_h
+                 and abs(b["rms"] - plain_best.rms) <= 5e-4 and abs(b["chi2"] - plain_best.chi2) <= 0.5)
+        out["screened"] = {"value": round(n_max * 10 / dt, 3), "unit": "pair-fits/s", "seconds": round(dt, 3),
+                           "keep": sc.keep, "kept_frequencies": int(kept.size),
+                           "refitted_frequencies": int(len(refitted)), "weights": "ps_claude_opt_amd/data/screen.npz",
+                           "best_line": b, "agrees_with_plain": bool(agree),
+                           "vs_truncated_cold": round(out["truncated_cold"]["seconds"] / dt, 3)}
+        print(f"# screened: {n_max * 10} pairs in {dt:.2f} s ({kept.size} of {n_max} frequencies fitted), best "
+              f"{b['period_h']:.6f} h rms {b['rms']:.6f} agrees {agree}", file=sys.stderr, flush=True)
     return out
 
 

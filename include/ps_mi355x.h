@@ -241,6 +241,20 @@ int ps_free_params(const ps_ctx* ctx, int* ia /* [ma] */);
 int ps_warm_n_features(void);
 int ps_warm_features(const ps_ctx* ctx, int n_items, const int* prob, const int* n_list, const int* m_list,
                      float* out /* [n_items][PS_WARM_FEATURES] */);
+/* Learned frequency screen (extension; the reference fits every trial
+ * frequency, start_CUDA.cu:792-1245).  Fold features of the trial
+ * frequencies n_first .. n_first + n_count - 1 (1-based) of problem `prob`,
+ * PS_SCREEN_FEATURES floats per frequency: every curve of at least 8 points,
+ * made relative (b / mean - 1), is folded at the trial frequency through
+ * its harmonic powers P_k = |sum y exp(2 pi i k nu t)|^2 / sum y^2, k = 1..4,
+ * summarised over the short (spanning at most 2 days) and the long curves,
+ * with the log trial period and the problem's curve and point counts.  The
+ * features do not depend on the pole.  Host code (up to 16 threads). */
+#define PS_SCREEN_HARMONICS 4
+#define PS_SCREEN_FEATURES 32
+int ps_screen_n_features(void);
+int ps_screen_features(const ps_ctx* ctx, int prob, int n_first, int n_count,
+                       float* out /* [n_count][PS_SCREEN_FEATURES] */);
 /* ps_fit_items starting from caller-given parameters: a0[k*ma + i] for the
  * shape coefficients (i < ncoef) and the phase-function / Lambert
  * parameters (i = ncoef+3 .. ncoef+6); the pole, rotation rate and the

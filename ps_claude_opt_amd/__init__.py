@@ -57,11 +57,20 @@ def write_output(path: str, lines: Sequence[Line]) -> None:
 _RESULT_FIELDS = ("reported", "niter", "dev", "dark", "period", "lambda_deg", "beta_deg", "chisq", "n_vis")
 
 
+RESULT_DTYPE = np.dtype([(f, "f8") for f in _RESULT_FIELDS])
+# ps_pair_result as laid out in memory (include/ps_mi355x.h)
+_C_RESULT = np.dtype([("reported", "i4"), ("niter", "i4"), ("dev", "f8"), ("dark", "f8"), ("period", "f8"),
+                      ("lambda_deg", "f8"), ("beta_deg", "f8"), ("chisq", "f8"), ("n_vis", "i8")])
+
+
 def _results(out, n: int) -> np.ndarray:
-    res = np.zeros(n, dtype=[(f, "f8") for f in _RESULT_FIELDS])
-    for i in range(n):
-        for f in _RESULT_FIELDS:
-            res[i][f] = getattr(out[i], f)
+    """The first n ps_pair_result entries of a ctypes array, as a float64
+    record array."""
+    assert C.sizeof(_abi.PsPairResult) == _C_RESULT.itemsize
+    raw = np.frombuffer(out, dtype=_C_RESULT, count=n)
+    res = np.zeros(n, dtype=RESULT_DTYPE)
+    for f in _RESULT_FIELDS:
+        res[f] = raw[f]
     return res
 
 
@@ -202,6 +211,15 @@ class Scan:
         out = np.zeros((len(p_arr), nf), np.float32)
         self._check(self._lib.ps_warm_features(self._h, len(p_arr), _abi.iptr(p_arr), _abi.iptr(n_arr),
                                                _abi.iptr(m_arr), _abi.fptr(out)))
+        return out
+
+    def screen_features(self, n_start: int, n_count: int, prob: int = 0) -> np.ndarray:
+        """Fold features of the learned frequency screen for frequencies
+        n_start .. n_start + n_count - 1 of problem `prob`
+        ([n_count][PS_SCREEN_FEATURES], FP32; ps_screen_features)."""
+        nf = self._lib.ps_screen_n_features()
+        out = np.zeros((max(n_count, 0), nf), np.float32)
+        self._check(self._lib.ps_screen_features(self._h, prob, n_start, n_count, _abi.fptr(out)))
         return out
 
     def initial_params(self, prob, n_list, m_list) -> np.ndarray:

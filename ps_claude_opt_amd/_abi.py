@@ -88,6 +88,8 @@ SIGNATURES = {
     "ps_free_params": (C.c_int, [C.c_void_p, _ip]),
     "ps_warm_n_features": (C.c_int, []),
     "ps_warm_features": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, C.POINTER(C.c_float)]),
+    "ps_screen_n_features": (C.c_int, []),
+    "ps_screen_features": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "ps_fit_items_from": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, C.c_int,
                                     C.POINTER(PsPairResult), _dp]),
     "ps_mlp_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,

@@ -296,14 +296,59 @@ def refit_set(rms: np.ndarray, refit: int = REFIT_MIN, band: float = REFIT_BAND)
     return np.sort(order[:k])
 
 
+# Period aliases of a lightcurve fit: a convex model fits a rotational
+# lightcurve about as well at these multiples of the period, and which of
+# them a plain scan ranks first is decided by small rms differences.
+ALIAS_MULTIPLES = (2.0, 0.5, 1.5, 2.0 / 3.0, 3.0, 1.0 / 3.0, 2.5, 0.4)
+ALIAS_SEEDS = 3   # the best truncated frequencies whose aliases are refitted too
+ALIAS_WIDTH = 3   # frequency steps either side of each alias
+
+
+def alias_windows(info: dict, n_list, width: int = ALIAS_WIDTH, mults=ALIAS_MULTIPLES) -> np.ndarray:
+    """The 1-based trial frequencies within `width` steps of each alias
+    (period times mults) of the frequencies n_list, on the problem's grid
+    f_start - (n - 1) f_step (1/h); sorted, unique, inside 1 .. n_max."""
+    out = set()
+    for n in np.atleast_1d(n_list):
+        period_h = 1.0 / (info["f_start"] - (int(n) - 1) * info["f_step"])
+        for m in mults:
+            k = int(round((info["f_start"] - 1.0 / (m * period_h)) / info["f_step"])) + 1
+            for j in range(k - width, k + width + 1):
+                if 1 <= j <= info["n_max"]:
+                    out.add(j)
+    return np.array(sorted(out), np.int64)
+
+
+def refit_frequencies(info: dict, rms: np.ndarray, n_start: int, refit: int = REFIT_MIN, band: float = REFIT_BAND,
+                      alias_seeds: int = ALIAS_SEEDS, alias_width: int = ALIAS_WIDTH, among=None) -> np.ndarray:
+    """Frequencies (n_start-based, sorted) to refit cold after a truncated
+    screen whose best truncated rms per frequency is `rms` (n_start ..;
+    NaN where a frequency was not fitted): refit_set's frequencies, plus
+    the alias windows of the `alias_seeds` best ones, within the scanned
+    range."""
+    r = np.asarray(rms, np.float64)
+    if among is not None:
+        sub = np.asarray(among, np.int64) - n_start
+        base = sub[refit_set(r[sub], refit, band)]
+    else:
+        base = refit_set(r, refit, band)
+    rr = np.where(np.isnan(r), np.inf, r)
+    order = np.argsort(rr, kind="stable")
+    seeds = [n_start + int(j) for j in order[:alias_seeds] if np.isfinite(rr[j])]
+    al = alias_windows(info, seeds, alias_width) if seeds and alias_seeds > 0 else np.zeros(0, np.int64)
+    al = al[(al >= n_start) & (al < n_start + r.size)] - n_start
+    return n_start + np.union1d(base, al).astype(np.int64)
+
+
 def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, n_start: int, n_count: int,
-              n_iter: int, refit: int = REFIT_MIN, band: float = REFIT_BAND):
+              n_iter: int, refit: int = REFIT_MIN, band: float = REFIT_BAND, alias_seeds: int = ALIAS_SEEDS):
     """Warm-started scan of problem 0: every pair runs n_iter LM iterations
     from the network's prediction (ws None: from the reference's start);
-    then the frequencies of refit_set (the `refit` lowest truncated rms and
-    all within `band` of the lowest) are fitted again cold, with the
-    problem's own stop condition (all 10 poles), and those cold results
-    replace the truncated ones.  Returns (results [n_count*10]
+    then the frequencies of refit_frequencies (the `refit` lowest truncated
+    rms, all within `band` of the lowest, and the alias windows of the
+    `alias_seeds` lowest) are fitted again cold, with the problem's own stop
+    condition (all 10 poles), and those cold results replace the truncated
+    ones.  Returns (results [n_count*10]
     frequency-major, refitted frequency indices).  The scan's winner is the
     best of the refitted frequencies (best_line(..., among=refitted)):
     those are plain fits, and a warm-started truncated fit may end below a
@@ -315,7 +360,7 @@ def warm_scan(scan, ws: Optional[WarmStart], net: Optional[Mlp], conw_r: float, 
     res, _ = scan.fit_items_from(p, n, m, conw_r, conw_r, a0, n_iter=n_iter)
     best = best_lines(res, n_count)
     rms = res["dev"].reshape(n_count, 10)[np.arange(n_count), best]
-    fr = refit_set(rms, refit, band)
+    fr = refit_frequencies(scan.info, rms, n_start, refit, band, alias_seeds) - n_start
     if fr.size:
         n2 = np.repeat(n_start + fr, 10)
         m2 = np.tile(np.arange(1, 11), fr.size)

@@ -31,13 +31,15 @@ import time
 import numpy as np
 
 
-def winner_ranks(scan, ws, net, conw_r: float, n_max: int, iters, dump=None):
+def winner_ranks(scan, ws, net, conw_r: float, n_max: int, iters, dump=None, lines=None, modes=("warm", "cold")):
     """(plain winner frequency index, {mode+n_iter: rank of the winner});
     dump (a dict) receives the per-frequency best rms of the plain scan
-    ("plain") and of every truncated screen ("warm20", "cold15", ...)."""
+    ("plain") and of every truncated screen ("warm20", "cold15", ...).
+    lines: the plain scan's lines if already run."""
     from . import warm
 
-    lines = scan.start(conw_r, 1, n_max)
+    if lines is None:
+        lines = scan.start(conw_r, 1, n_max)
     rms = np.array([L.rms for L in lines])
     f_best = int(np.nanargmin(rms))
     if dump is not None:
@@ -45,9 +47,9 @@ def winner_ranks(scan, ws, net, conw_r: float, n_max: int, iters, dump=None):
     n = np.repeat(np.arange(1, n_max + 1), 10)
     m = np.tile(np.arange(1, 11), n_max)
     p = np.zeros(n.size, np.int32)
-    a0w = ws.initial_params(scan, net, p, n, m)
+    a0w = ws.initial_params(scan, net, p, n, m) if "warm" in modes else None
     ranks = {}
-    for mode, a0 in (("warm", a0w), ("cold", None)):
+    for mode, a0 in [x for x in (("warm", a0w), ("cold", None)) if x[0] in modes]:
         for it in iters:
             res, _ = scan.fit_items_from(p, n, m, conw_r, conw_r, a0, n_iter=it)
             best = warm.best_lines(res, n_max)
@@ -62,6 +64,30 @@ def winner_ranks(scan, ws, net, conw_r: float, n_max: int, iters, dump=None):
     return f_best, ranks
 
 
+def screen_check(scan, sc, net, conw_r: float, n_max: int, lines, n_iter: int = 20, dump=None) -> dict:
+    """The learned screen on one asteroid whose plain scan gave `lines`:
+    the plain winner's rank under the screen's score, whether the kept set
+    holds it, and whether the screened scan (screen.screened_scan) reports
+    the plain winner's line."""
+    from . import screen, warm
+
+    rms = np.array([L.rms for L in lines])
+    f_best = int(np.nanargmin(rms))
+    scores = sc.scores(scan, net, 1, n_max)
+    if dump is not None:
+        dump["screen"] = scores
+    t0 = time.perf_counter()
+    res, refitted, kept = screen.screened_scan(scan, sc, net, conw_r, 1, n_max, n_iter=n_iter)
+    dt = time.perf_counter() - t0
+    b = warm.best_line(res, 1, n_max, among=refitted)
+    L = lines[f_best]
+    agrees = (b["freq_index"] == f_best + 1 and b["rms"] == L.rms and b["lambda_deg"] == L.lambda_deg
+              and b["beta_deg"] == L.beta_deg)
+    return dict(winner_rank=screen.rank_of(scores, f_best), kept=int(kept.size),
+                in_kept=bool(f_best + 1 in set(kept.tolist())), refitted=int(refitted.size),
+                screened_winner=b["freq_index"], agrees=bool(agrees), screened_s=round(dt, 3))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--asteroids", type=int, default=16)
@@ -72,11 +98,17 @@ def main(argv=None) -> int:
     ap.add_argument("--weights", default=None)
     ap.add_argument("--first", type=int, default=0, help="index of the first held-out asteroid")
     ap.add_argument("--dump", default=None, help="npz of per-frequency best rms of every scan")
+    ap.add_argument("--modes", default="warm,cold,screen",
+                    help="warm / cold: truncated screens from the network's and the reference's start; "
+                         "screen: the learned frequency screen")
+    ap.add_argument("--screen", default=None, help="screen weights (default screen.DEFAULT_WEIGHTS)")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import ps_claude_opt_amd as ps
-    from ps_claude_opt_amd import synth, warm
+    from ps_claude_opt_amd import screen, synth, warm
 
+    modes = set(args.modes.split(","))
+    sc = screen.Screen.load(args.screen or screen.DEFAULT_WEIGHTS) if "screen" in modes else None
     iters = [int(x) for x in args.iters.split(",")]
     refits = [int(x) for x in args.refit.split(",")]
     ws = warm.WarmStart.load(args.weights or warm.DEFAULT_WEIGHTS)
@@ -91,18 +123,32 @@ def main(argv=None) -> int:
         with ps.Scan(0) as s:
             s.prepare_file(path)
             cw = s.conw_search()
-            net = ws.network(s)
-            f_best, ranks = winner_ranks(s, ws, net, cw, s.info["n_max"], iters, d)
-            net.close()
+            row = dict(asteroid=i)
+            lines = s.start(cw, 1, s.info["n_max"])
+            if modes & {"warm", "cold"}:
+                net = ws.network(s) if "warm" in modes else None
+                f_best, ranks = winner_ranks(s, ws, net, cw, s.info["n_max"], iters, d, lines, modes)
+                if net is not None:
+                    net.close()
+                row.update(winner_freq=f_best + 1, ranks=ranks)
+            if sc is not None:
+                snet = sc.network(s)
+                row["screen"] = screen_check(s, sc, snet, cw, s.info["n_max"], lines, dump=d)
+                snet.close()
+                row.setdefault("winner_freq", int(np.nanargmin([L.rms for L in lines])) + 1)
         if d is not None:
+            d["plain"] = np.array([L.rms for L in lines])
             dumps.update({f"a{i}_{k}": v for k, v in d.items()})
-        rows.append(dict(asteroid=i, winner_freq=f_best + 1, ranks=ranks, seconds=round(time.time() - t0, 1)))
+        row["seconds"] = round(time.time() - t0, 1)
+        rows.append(row)
         print(json.dumps(rows[-1]), flush=True)
     if args.dump:
         os.makedirs(os.path.dirname(os.path.abspath(args.dump)), exist_ok=True)
         np.savez_compressed(args.dump, **dumps)
     summary = {}
     for mode in ("warm", "cold"):
+        if mode not in modes:
+            continue
         for it in iters:
             for rf in refits:
                 ok = sum(r["ranks"][f"{mode}{it}"] < rf for r in rows)
@@ -111,7 +157,17 @@ def main(argv=None) -> int:
             summary[f"{mode} n_iter={it} adaptive refit set"] = (
                 f"{sum(a[0] for a in ad)}/{len(rows)}, mean {np.mean([a[1] for a in ad]):.0f} "
                 f"max {max(a[1] for a in ad)} frequencies refitted")
-    report = dict(asteroids=args.asteroids, iters=iters, refits=refits, rows=rows, found_winner=summary)
+    if sc is not None:
+        sr = [r["screen"] for r in rows]
+        summary["screen"] = dict(
+            keep=sc.keep, kept=sr[0]["kept"] if sr else 0,
+            plain_winner_kept=f"{sum(x['in_kept'] for x in sr)}/{len(sr)}",
+            screened_scan_reports_plain_winner=f"{sum(x['agrees'] for x in sr)}/{len(sr)}",
+            winner_rank_max=max(x["winner_rank"] for x in sr), winner_rank_median=float(np.median([x["winner_rank"] for x in sr])),
+            refitted_mean=float(np.mean([x["refitted"] for x in sr])), refitted_max=max(x["refitted"] for x in sr),
+            screened_s_mean=float(np.mean([x["screened_s"] for x in sr])))
+    report = dict(asteroids=args.asteroids, first=args.first, modes=sorted(modes), iters=iters, refits=refits,
+                  rows=rows, found_winner=summary)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:
         json.dump(report, f, indent=1)

@@ -62,9 +62,10 @@ def generate(directory: str, first: int, count: int, workers: int, preset: str =
 
 def true_freq_index(scan, periods_h) -> np.ndarray:
     """Each problem's scan frequency nearest to its true rotation frequency
-    (the scan's periods are 1 / (f_start - (n - 1) f_step) days)."""
-    return np.array([int(round((b["f_start"] - 24.0 / p) / b["f_step"])) + 1
-                     for b, p in zip(scan.batch_info, periods_h)])
+    (the scan's periods are 1 / (f_start - (n - 1) f_step) hours)."""
+    from ps_claude_opt_amd import screen
+
+    return np.array([screen.true_index(b, p) for b, p in zip(scan.batch_info, periods_h)])
 
 
 def initial_conw(scan) -> np.ndarray:
