@@ -83,7 +83,7 @@ def main() -> int:
     ap.add_argument("--warm-iters", type=int, default=20, help="LM iterations per pair of the warm-started scan")
     ap.add_argument("--warm-refit", type=int, default=10,
                     help="frequencies of the warm-started scan always refitted with full cold fits")
-    ap.add_argument("--warm-band", type=float, default=0.10,
+    ap.add_argument("--warm-band", type=float, default=0.15,
                     help="also refit every frequency whose truncated rms is within this fraction of the best")
     ap.add_argument("--work-dir", default=os.path.join(ROOT, "gpurun_out", "bench"))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

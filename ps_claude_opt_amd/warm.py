@@ -279,7 +279,7 @@ def best_lines(res: np.ndarray, n_freq: int):
 
 
 REFIT_MIN = 10      # frequencies always refitted cold
-REFIT_BAND = 0.10   # and every frequency whose truncated rms is within 10 % of the best one
+REFIT_BAND = 0.15   # and every frequency whose truncated rms is within 15 % of the best one
 
 
 def refit_set(rms: np.ndarray, refit: int = REFIT_MIN, band: float = REFIT_BAND) -> np.ndarray:

@@ -59,7 +59,7 @@ def winner_ranks(scan, ws, net, conw_r: float, n_max: int, iters, dump=None, lin
             order = np.argsort(np.where(np.isnan(r), np.inf, r), kind="stable")
             ranks[f"{mode}{it}"] = int(np.where(order == f_best)[0][0])
             # the adaptive refit set of warm.warm_scan: found, and its size
-            fr = warm.refit_set(r)
+            fr = warm.refit_frequencies(scan.info, r, 1) - 1
             ranks[f"{mode}{it}_adaptive"] = [bool(f_best in set(fr.tolist())), int(fr.size)]
     return f_best, ranks
 

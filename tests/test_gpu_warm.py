@@ -269,7 +269,7 @@ def test_warm_scan_refits_are_plain_fits(inputs):
         plain, _ = s.fit_items_from(p, n, m, cw, cw, None)
         ws = warm.WarmStart.load(warm.DEFAULT_WEIGHTS)
         net = ws.network(s)
-        res, fr = warm.warm_scan(s, ws, net, cw, 1, n_max, n_iter=4, refit=12, alias_seeds=0)
+        res, fr = warm.warm_scan(s, ws, net, cw, 1, n_max, n_iter=4, refit=12, band=0.0, alias_seeds=0)
         net.close()
     assert fr.size == 12
     rows = ((fr - 1)[:, None] * 10 + np.arange(10)).ravel()
