@@ -1150,10 +1150,12 @@ __device__ __forceinline__ void wave_argmax(unsigned long long& k, int& i) {
   k = ((unsigned long long)hi << 32) | lo;
 }
 
-// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) by the whole
-// workgroup on the augmented matrix [A | b] in LDS (rows 0..mf-1, stride
-// K = mf + 1, b in column mf).  Lane l of every wave owns columns l and
-// l + 64; wave w owns the rows ll = w (mod NW).  NR gaussj pivot rule: the
+// Damped full-pivot Gauss-Jordan (Start.cu:2838-3021) by the first GW
+// waves of the workgroup on the augmented matrix [A | b] in LDS (rows
+// 0..mf-1, stride K = mf + 1, b in column mf); further waves only keep the
+// barriers (a pivot search in more waves costs more issue slots than it
+// saves).  Lane l of every solving wave owns columns l and l + 64; wave w
+// owns the rows ll = w (mod GW).  NR gaussj pivot rule: the
 // largest |entry| among unpivoted rows and columns, the last occurrence in
 // row-major order on ties; each wave searches its rows while eliminating
 // them, reduces its lanes' candidates with DPP, and the waves' candidates
@@ -1164,10 +1166,13 @@ __device__ __forceinline__ void wave_argmax(unsigned long long& k, int& i) {
 // Returns 0, or 2 for a zero pivot (or nothing selectable); the solution is
 // left in column mf.  Every thread of the workgroup must call it.
 // TWO: columns beyond 64 exist (K > 64).
+constexpr int GW = NW > 4 ? 4 : NW;  // waves that solve
+
 template <bool TWO>
 __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* red) {
   const int tid = tid_opaque(), lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, in a scalar register
+  const int w0 = w < GW ? w : mf;                          // first owned row (none beyond GW)
   const int c0 = lane, c1 = lane + 64;
   const bool h0 = c0 < K, h1 = TWO && c1 < K;
   unsigned long long u0 = 0, u1 = 0;  // pivoted rows / columns
@@ -1181,7 +1186,7 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       bidx = idx;
     }
   };
-  for (int ll = w; ll < mf; ll += NW) {
+  for (int ll = w0; ll < mf; ll += GW) {
     if (c0 < mf) consider(a[ll * K + c0], ll * 128 + c0);
     if (c1 < mf) consider(a[ll * K + c1], ll * 128 + c1);
   }
@@ -1191,22 +1196,22 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
     wave_argmax(key, kidx);
     unsigned long long* ured = reinterpret_cast<unsigned long long*>(red);
     long long* ired = reinterpret_cast<long long*>(red + NW);
-    if (lane == 0) {
+    if (lane == 0 && w < GW) {
       ured[w] = key;
       ired[w] = kidx;
     }
     __syncthreads();
-    unsigned long long wk[NW];
-    int wi[NW];
+    unsigned long long wk[GW];
+    int wi[GW];
 #pragma unroll
-    for (int ww = 0; ww < NW; ++ww) {
+    for (int ww = 0; ww < GW; ++ww) {
       wk[ww] = ured[ww];
       wi[ww] = (int)ired[ww];
     }
     key = 0;
     kidx = -1;
 #pragma unroll
-    for (int ww = 0; ww < NW; ++ww)
+    for (int ww = 0; ww < GW; ++ww)
       if (cand_better(wk[ww], wi[ww], key, kidx)) {
         key = wk[ww];
         kidx = wi[ww];
@@ -1252,22 +1257,22 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
     const double pn1 = (c1 == icol ? 1.0 : pv1) * pivinv;
     // the row swap: row icol becomes the scaled pivot row, row irow old
     // row icol (each written by the wave that owns it)
-    if (icol % NW == w) {
+    if (icol % GW == w) {
       if (h0) a[icol * K + c0] = pn0;
       if (h1) a[icol * K + c1] = pn1;
     }
-    if (irow != icol && irow % NW == w) {
+    if (irow != icol && irow % GW == w) {
       if (h0) a[irow * K + c0] = cv0;
       if (h1) a[irow * K + c1] = cv1;
     }
     big = 0.0;
     bidx = -1;
     // eliminate this wave's rows
-    for (int l0 = w; l0 < mf; l0 += 16 * NW) {
+    for (int l0 = w0; l0 < mf; l0 += 16 * GW) {
       double dm[16], b0[16], b1[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
-        const int ll = l0 + u * NW;
+        const int ll = l0 + u * GW;
         const int lr = ll < mf ? ll : w;
         dm[u] = a[lr * K + icol];
         b0[u] = h0 ? a[lr * K + c0] : 0.0;
@@ -1275,7 +1280,7 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
-        const int ll = l0 + u * NW;
+        const int ll = l0 + u * GW;
         if (ll >= mf || ll == icol) continue;
         const double dum = dm[u];
         const bool cand = !used(ll);
@@ -1496,14 +1501,16 @@ __device__ __forceinline__ void fit_pair(const PsFitArgs& A, const PsDevProblem&
 // Register budget: the resident wavefronts per SIMD the allocation targets.
 // Up to 64 Jacobian columns a 4-wave fit fits in 168 VGPRs (three
 // workgroups per CU; the few spills are in per-iteration bookkeeping, not in
-// the facet, SYRK or Gauss-Jordan loops); wider fits and 8-wave workgroups
-// take up to 256 (two waves per SIMD).  PS_FIT_WAVES_PER_EU overrides it.
+// the facet, SYRK or Gauss-Jordan loops) and an 8-wave fit in 128 (two
+// workgroups per CU, four waves per SIMD: each wave holds half of its
+// chain's block pairs); wider fits take up to 256 (two waves per SIMD).
+// PS_FIT_WAVES_PER_EU overrides it.
 template <int NB>
 constexpr int fit_waves_per_eu() {
 #ifdef PS_FIT_WAVES_PER_EU
   return PS_FIT_WAVES_PER_EU;
 #else
-  return NW == 4 && NB <= 4 ? 3 : 2;  // 1 and 2 waves per pair: LDS limits the workgroups per CU
+  return NB <= 4 ? (NW == 4 ? 3 : NW == 8 ? 4 : 2) : 2;  // 1 and 2 waves per pair: LDS limits the workgroups per CU
 #endif
 }
 
