@@ -102,6 +102,7 @@ def main(argv=None) -> int:
                     help="warm / cold: truncated screens from the network's and the reference's start; "
                          "screen: the learned frequency screen")
     ap.add_argument("--screen", default=None, help="screen weights (default screen.DEFAULT_WEIGHTS)")
+    ap.add_argument("--extra-screens", default="", help="comma-separated screen weights whose scores are dumped too")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import ps_claude_opt_amd as ps
@@ -134,6 +135,12 @@ def main(argv=None) -> int:
             if sc is not None:
                 snet = sc.network(s)
                 row["screen"] = screen_check(s, sc, snet, cw, s.info["n_max"], lines, dump=d)
+                for j, ep in enumerate(x for x in args.extra_screens.split(",") if x):
+                    esc = screen.Screen.load(ep)
+                    enet = esc.network(s)
+                    if d is not None:
+                        d[f"screen_x{j}"] = esc.scores(s, enet, 1, s.info["n_max"])
+                    enet.close()
                 snet.close()
                 row.setdefault("winner_freq", int(np.nanargmin([L.rms for L in lines])) + 1)
         if d is not None:
