@@ -23,7 +23,10 @@
 //
 // --device N selects the GPU unless a BOINC client's init_data.xml names
 // one (gpu_device_num, period_search_BOINC.cpp:598-614).  --N k sets the
-// batch to 250 k frequencies (default k = 2), --batch F to F frequencies.
+// batch to 250 k frequencies, --batch F to F frequencies.  The default,
+// k = 10, fits 2,500 frequencies (25,000 pair-fits) per launch: smaller
+// launches end with a longer tail in which some workgroups have no pair
+// left (about 4 % of the time at 500 frequencies).
 // --max-batches K stops after K scan batches, leaving a resumable state
 // (for time-sliced runs).
 // Exit status: 0 done, 1 output not writable, 2 bad input, 3 device
@@ -82,7 +85,7 @@ std::string dir_of(const std::string& p) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  int device = -1, nfactor = 2, max_batches = 0, batch_freqs = 0;
+  int device = -1, nfactor = 10, max_batches = 0, batch_freqs = 0;
   std::string in_name = "period_search_in", out_name = "period_search_out", state_path;
   int pos = 0;
   for (int i = 1; i < argc; ++i) {
