@@ -158,6 +158,50 @@ def screened_scan(scan, sc: Screen, net: warm.Mlp, conw_r: float, n_start: int, 
     return res, n_start + fr, n_start + kept
 
 
+def screened_scan_batch(scan, sc: Screen, net: warm.Mlp, conw_r, n_iter: int, refit: int = warm.REFIT_MIN,
+                        band: float = warm.REFIT_BAND, alias_seeds: int = warm.ALIAS_SEEDS):
+    """screened_scan over every problem of a prepared batch (the whole grid
+    of each), with the truncated fits of all problems in one persistent
+    launch and their cold refits in a second one.  conw_r: one value or one
+    per problem.  Returns, per problem, (results [n_max*10], refitted
+    1-based frequencies, kept 1-based frequencies)."""
+    from . import RESULT_DTYPE
+
+    infos = scan.batch_info
+    cw = np.broadcast_to(np.asarray(conw_r, np.float64), (len(infos),))
+    kept = []
+    for i, b in enumerate(infos):
+        scores = sc.scores(scan, net, 1, b["n_max"], prob=i)
+        kept.append(keep_set(scores, sc.n_keep(b["n_max"])))
+    p = np.concatenate([np.full(k.size * 10, i, np.int32) for i, k in enumerate(kept)])
+    n = np.concatenate([np.repeat(k + 1, 10) for k in kept])
+    m = np.concatenate([np.tile(np.arange(1, 11), k.size) for k in kept])
+    part, _ = scan.fit_items_from(p, n, m, cw[p], cw[p], None, n_iter=n_iter)
+    out, refits, off = [], [], 0
+    for i, (b, k) in enumerate(zip(infos, kept)):
+        res = np.zeros(b["n_max"] * 10, dtype=RESULT_DTYPE)
+        res["dev"] = np.nan
+        pk = part[off:off + k.size * 10]
+        off += k.size * 10
+        res[(np.repeat(k, 10) * 10 + np.tile(np.arange(10), k.size))] = pk
+        best = warm.best_lines(pk, k.size)
+        rms = np.full(b["n_max"], np.nan)
+        rms[k] = pk["dev"].reshape(k.size, 10)[np.arange(k.size), best]
+        refits.append(warm.refit_frequencies(b, rms, 1, refit, band, alias_seeds, among=k + 1))
+        out.append(res)
+    p2 = np.concatenate([np.full(f.size * 10, i, np.int32) for i, f in enumerate(refits)])
+    n2 = np.concatenate([np.repeat(f, 10) for f in refits])
+    m2 = np.concatenate([np.tile(np.arange(1, 11), f.size) for f in refits])
+    if p2.size:
+        cold, _ = scan.fit_items_from(p2, n2, m2, cw[p2], cw[p2], None)
+        off = 0
+        for i, f in enumerate(refits):
+            for j, fr in enumerate(f):
+                out[i][(fr - 1) * 10:fr * 10] = cold[off + j * 10:off + (j + 1) * 10]
+            off += f.size * 10
+    return [(out[i], refits[i], kept[i] + 1) for i in range(len(infos))]
+
+
 def true_index(info: dict, period_h: float) -> int:
     """1-based index of the trial frequency nearest to 1 / period_h on the
     problem's grid f_start - (n - 1) f_step (1/h)."""

@@ -88,6 +88,39 @@ def screen_check(scan, sc, net, conw_r: float, n_max: int, lines, n_iter: int = 
                 screened_winner=b["freq_index"], agrees=bool(agrees), screened_s=round(dt, 3))
 
 
+def screen_batch_run(paths, sc, n_iter: int = 20) -> list:
+    """One prepared batch of asteroids: the plain batched scan and the
+    batched screened scan, both timed; per asteroid whether the screened
+    scan reports the plain winner's line."""
+    import ps_claude_opt_amd as ps
+    from . import screen, warm
+
+    with ps.Scan(0) as s:
+        s.prepare_files(paths)
+        cw = s.conw_search_batch()
+        t0 = time.perf_counter()
+        plain = s.start_batch(cw)
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        net = sc.network(s)
+        got = screen.screened_scan_batch(s, sc, net, cw, n_iter=n_iter)
+        net.close()
+        t_screen = time.perf_counter() - t0
+    pairs = sum(10 * len(L) for L in plain)
+    rows = []
+    for i, (res, refitted, kept) in enumerate(got):
+        rms = np.array([L.rms for L in plain[i]])
+        f_best = int(np.nanargmin(rms))
+        b = warm.best_line(res, 1, len(plain[i]), among=refitted)
+        L = plain[i][f_best]
+        rows.append(dict(winner_freq=f_best + 1, kept=bool(f_best + 1 in set(kept.tolist())), refitted=int(refitted.size),
+                         agrees=bool(b["freq_index"] == f_best + 1 and b["rms"] == L.rms
+                                     and b["lambda_deg"] == L.lambda_deg and b["beta_deg"] == L.beta_deg)))
+    return [dict(asteroids=len(paths), pairs=pairs, plain_s=round(t_plain, 3), screened_s=round(t_screen, 3),
+                 plain_pair_fits_per_s=round(pairs / t_plain, 1), screened_pair_fits_per_s=round(pairs / t_screen, 1),
+                 rows=rows)]
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--asteroids", type=int, default=16)
@@ -103,19 +136,43 @@ def main(argv=None) -> int:
                          "screen: the learned frequency screen")
     ap.add_argument("--screen", default=None, help="screen weights (default screen.DEFAULT_WEIGHTS)")
     ap.add_argument("--extra-screens", default="", help="comma-separated screen weights whose scores are dumped too")
+    ap.add_argument("--batch-size", type=int, default=16, help="asteroids per prepared batch (--modes screen_batch)")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import ps_claude_opt_amd as ps
     from ps_claude_opt_amd import screen, synth, warm
 
     modes = set(args.modes.split(","))
-    sc = screen.Screen.load(args.screen or screen.DEFAULT_WEIGHTS) if "screen" in modes else None
+    sc = screen.Screen.load(args.screen or screen.DEFAULT_WEIGHTS) if modes & {"screen", "screen_batch"} else None
     iters = [int(x) for x in args.iters.split(",")]
     refits = [int(x) for x in args.refit.split(",")]
     ws = warm.WarmStart.load(args.weights or warm.DEFAULT_WEIGHTS)
     os.makedirs(args.work_dir, exist_ok=True)
     rows = []
     dumps = {}
+    if modes == {"screen_batch"}:
+        batches = []
+        for b0 in range(args.first, args.first + args.asteroids, args.batch_size):
+            paths = []
+            for i in range(b0, min(args.first + args.asteroids, b0 + args.batch_size)):
+                path = os.path.join(args.work_dir, f"asteroid_{i}.in")
+                synth.write_asteroid(path, "c2", synth.config_seed(2, 300 + i))
+                paths.append(path)
+            batches += screen_batch_run(paths, sc)
+            print(json.dumps({k: v for k, v in batches[-1].items() if k != "rows"}), flush=True)
+        allr = [r for b in batches for r in b["rows"]]
+        summary = dict(screen_batch=dict(
+            asteroids=len(allr), batch_size=args.batch_size, keep=sc.keep,
+            screened_scan_reports_plain_winner=f"{sum(r['agrees'] for r in allr)}/{len(allr)}",
+            plain_pair_fits_per_s=round(sum(b["pairs"] for b in batches) / sum(b["plain_s"] for b in batches), 1),
+            screened_pair_fits_per_s=round(sum(b["pairs"] for b in batches) / sum(b["screened_s"] for b in batches), 1)))
+        report = dict(asteroids=args.asteroids, first=args.first, modes=["screen_batch"], batches=batches,
+                      found_winner=summary)
+        os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+        with open(args.out, "w") as f:
+            json.dump(report, f, indent=1)
+        print(json.dumps(summary, indent=1), flush=True)
+        return 0
     for i in range(args.first, args.first + args.asteroids):
         path = os.path.join(args.work_dir, f"asteroid_{i}.in")
         synth.write_asteroid(path, "c2", synth.config_seed(2, 300 + i))

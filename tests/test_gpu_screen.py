@@ -134,3 +134,41 @@ def test_screened_scan_refits_are_plain_fits(tmp_path):
     # the kept set) were not fitted
     idle = np.setdiff1d(np.setdiff1d(np.arange(1, n_max + 1), kept), refitted)
     assert idle.size > 0 and np.all(np.isnan(res["dev"].reshape(n_max, 10)[idle - 1]))
+
+
+@pytest.mark.skipif(not os.path.exists(screen.DEFAULT_WEIGHTS), reason="no trained screen shipped")
+def test_screened_batch_matches_plain_and_single_scans(tmp_path):
+    """A batch of four config-2-style asteroids (the held-out evaluation's
+    first four): the batched screened scan reports each asteroid's plain
+    winner line, and asteroid 0's results equal the single-asteroid
+    screened scan's bit for bit."""
+    paths = []
+    for i in range(4):
+        p = str(tmp_path / f"a{i}.in")
+        synth.write_asteroid(p, "c2", synth.config_seed(2, 300 + i))
+        paths.append(p)
+    sc = screen.Screen.load()
+    with ps.Scan(0) as s:
+        s.prepare_files(paths)
+        cw = s.conw_search_batch()
+        plain = s.start_batch(cw)
+        net = sc.network(s)
+        got = screen.screened_scan_batch(s, sc, net, cw, n_iter=20)
+        net.close()
+    for i, (res, refitted, kept) in enumerate(got):
+        n_max = len(plain[i])
+        rms = np.array([L.rms for L in plain[i]])
+        f_best = int(np.nanargmin(rms)) + 1
+        b = warm.best_line(res, 1, n_max, among=refitted)
+        L = plain[i][f_best - 1]
+        assert (b["freq_index"], b["rms"], b["lambda_deg"], b["beta_deg"]) == (f_best, L.rms, L.lambda_deg, L.beta_deg), i
+        assert kept.size == sc.n_keep(n_max)
+    with ps.Scan(0) as s:
+        s.prepare_file(paths[0])
+        net = sc.network(s)
+        res0, fr0, kept0 = screen.screened_scan(s, sc, net, float(cw[0]), 1, s.info["n_max"], n_iter=20)
+        net.close()
+    res_b, fr_b, kept_b = got[0]
+    assert np.array_equal(kept0, kept_b) and np.array_equal(fr0, fr_b)
+    for f in ("dev", "chisq", "lambda_deg", "beta_deg", "period", "niter"):
+        assert np.array_equal(res0[f], res_b[f], equal_nan=True), f
