@@ -38,7 +38,7 @@ from . import warm
 
 N_HIDDEN = 128
 N_LAYERS = 2
-KEEP = 0.45      # fraction of the scan's frequencies the screened scan fits
+KEEP = 0.50      # fraction of the scan's frequencies the screened scan fits
 KEEP_MIN = 50    # and at least this many
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "screen.npz")
 
