@@ -292,14 +292,18 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
 
 def autotune(scan, probs, n_max, conw, skip: bool) -> dict:
     """Picks the launch configuration (wavefronts per pair, workgroups per
-    CU) on a seeded sample of 4096 of the workload's pairs, before the timed
-    region.  The configurations give bit-identical results."""
+    CU, facet-pass claim) on a seeded sample of 12,288 of the workload's
+    pairs, before the timed region.  The configurations give bit-identical
+    results."""
     import numpy as np
 
     if skip:
         return dict(scan.get_launch(), tuned=False)
     rng = np.random.default_rng(777)
-    k = 4096  # about 8 pairs per resident workgroup, so the launch tail does not decide
+    # a whole number of rounds of pairs for every launch shape on 256 CUs (2,
+    # 3, 4, 6, 8 or 12 resident workgroups per CU all divide 12,288), so that
+    # no shape gains or loses from the size of its last round
+    k = 12288
     pi = rng.integers(0, len(probs), k)
     p = np.asarray(probs)[pi]
     nl = np.array([rng.integers(1, int(np.asarray(n_max)[j]) + 1) for j in pi])

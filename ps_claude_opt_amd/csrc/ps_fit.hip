@@ -1134,6 +1134,50 @@ __device__ __forceinline__ void cand_step(unsigned long long& k, int& i) {
   }
 }
 
+// One step of a wave maximum of a double (lanes outside ROWMASK keep their
+// own value).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double max_step(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xF, false);
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xF, false);
+  return fmax(v, __longlong_as_double(((long long)(unsigned)ohi << 32) | (unsigned)olo));
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int imax_step(int v) {
+  const int o = __builtin_amdgcn_update_dpp(v, v, CTRL, ROWMASK, 0xF, false);
+  return o > v ? o : v;
+}
+
+// The wave's best pivot candidate, as wave_argmax, in two plain maxima: the
+// largest |entry| (t; -1 where a lane has no candidate), then the largest
+// position among the lanes that hold it (the NR rule's last occurrence).
+// Both in lane 63 after the same DPP pattern, read back to every lane; a
+// third of wave_argmax's instructions, with no compare-and-select chains.
+__device__ __forceinline__ void wave_argmax2(double t, int idx, unsigned long long& k, int& i) {
+  double m = max_step<0xB1, 0xF>(t);
+  m = max_step<0x4E, 0xF>(m);
+  m = max_step<0x141, 0xF>(m);
+  m = max_step<0x140, 0xF>(m);
+  m = max_step<0x142, 0xA>(m);
+  m = max_step<0x143, 0xC>(m);
+  const unsigned long long mb = __double_as_longlong(m);
+  const unsigned mlo = __builtin_amdgcn_readlane((int)(unsigned)mb, 63);
+  const unsigned mhi = __builtin_amdgcn_readlane((int)(unsigned)(mb >> 32), 63);
+  const double M = __longlong_as_double(((long long)mhi << 32) | mlo);
+  int j = t == M ? idx : -1;
+  j = imax_step<0xB1, 0xF>(j);
+  j = imax_step<0x4E, 0xF>(j);
+  j = imax_step<0x141, 0xF>(j);
+  j = imax_step<0x140, 0xF>(j);
+  j = imax_step<0x142, 0xA>(j);
+  j = imax_step<0x143, 0xC>(j);
+  i = M < 0.0 ? -1 : __builtin_amdgcn_readlane(j, 63);
+  k = M < 0.0 ? 0ull : (unsigned long long)__double_as_longlong(M);
+}
+
 // The wave's best candidate, in lane 63: quad butterflies, the half-row and
 // row mirrors (every lane of a 16-lane row then holds the row's best), then
 // the row broadcasts 15 (rows 1, 3) and 31 (rows 2, 3).
@@ -1191,9 +1235,16 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
     if (c1 < mf) consider(a[ll * K + c1], ll * 128 + c1);
   }
   for (int i = 0; i < mf; ++i) {
-    unsigned long long key = bidx >= 0 ? (unsigned long long)__double_as_longlong(big) : 0ull;
-    int kidx = bidx;
-    wave_argmax(key, kidx);
+    unsigned long long key;
+    int kidx;
+    if constexpr (NW <= 4) {
+      wave_argmax2(bidx >= 0 ? big : -1.0, bidx, key, kidx);
+    } else {
+      // the 8-wave kernel (128 VGPRs) ran 1.1 % slower with the two maxima
+      key = bidx >= 0 ? (unsigned long long)__double_as_longlong(big) : 0ull;
+      kidx = bidx;
+      wave_argmax(key, kidx);
+    }
     unsigned long long* ured = reinterpret_cast<unsigned long long*>(red);
     long long* ired = reinterpret_cast<long long*>(red + NW);
     if (lane == 0 && w < GW) {
