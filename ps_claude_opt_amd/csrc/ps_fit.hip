@@ -1136,6 +1136,14 @@ __device__ __forceinline__ void cand_step(unsigned long long& k, int& i) {
 
 // One step of a wave maximum of a double (lanes outside ROWMASK keep their
 // own value).
+// b, through a register the compiler cannot see into (keeps a per-lane flag
+// from being merged into the per-row uniform tests that use it)
+__device__ __forceinline__ bool opaque_flag(bool b) {
+  int v = b;
+  asm volatile("" : "+v"(v));
+  return v != 0;
+}
+
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ double max_step(double v) {
   const unsigned long long b = __double_as_longlong(v);
@@ -1219,29 +1227,41 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
   const int w0 = w < GW ? w : mf;                          // first owned row (none beyond GW)
   const int c0 = lane, c1 = lane + 64;
   const bool h0 = c0 < K, h1 = TWO && c1 < K;
+  // byte addresses of the elimination's loads and stores, row lr at
+  // e0 + lr * rs0: a lane with no column of the matrix (c >= K) loads and
+  // stores a dummy slot in row mf, which the solve never reads, instead of
+  // branching around each row's load and store
+  char* const ab = reinterpret_cast<char*>(a);
+  const unsigned rs0 = h0 ? 8u * K : 0u, rs1 = h1 ? 8u * K : 0u;
+  char* const e0 = ab + 8 * (h0 ? c0 : mf * K + min(c0 - K, K - 1));
+  char* const e1 = ab + 8 * (h1 ? c1 : mf * K + min(c1 - K, K - 1));
   unsigned long long u0 = 0, u1 = 0;  // pivoted rows / columns
-  auto used = [&](int k) -> bool { return k < 64 ? (u0 >> k) & 1ull : (u1 >> (k - 64)) & 1ull; };
-  double big = 0.0;
-  int bidx = -1;  // row * 128 + column
-  auto consider = [&](double v, int idx) {
-    const double t = fabs(v);
-    if (t >= big) {
-      big = t;
-      bidx = idx;
-    }
+  auto used = [&](int k) -> bool {
+    if constexpr (TWO)
+      return k < 64 ? (u0 >> k) & 1ull : (u1 >> (k - 64)) & 1ull;
+    else
+      return (u0 >> k) & 1ull;  // every index is below mf <= 63
+  };
+  double big = 0.0;  // the candidate's signed value: compared by magnitude
+  int bidx = -1;     // row * 128 + column
+  // one mask operation and one select per register (no nested selects)
+  auto consider = [&](bool ok, double v, int idx) {
+    const bool take = ok & (fabs(v) >= fabs(big));
+    big = take ? v : big;
+    bidx = take ? idx : bidx;
   };
   for (int ll = w0; ll < mf; ll += GW) {
-    if (c0 < mf) consider(a[ll * K + c0], ll * 128 + c0);
-    if (c1 < mf) consider(a[ll * K + c1], ll * 128 + c1);
+    consider(c0 < mf, *reinterpret_cast<const double*>(e0 + __umul24(ll, rs0)), ll * 128 + c0);
+    if constexpr (TWO) consider(c1 < mf, *reinterpret_cast<const double*>(e1 + __umul24(ll, rs1)), ll * 128 + c1);
   }
   for (int i = 0; i < mf; ++i) {
     unsigned long long key;
     int kidx;
     if constexpr (NW <= 4) {
-      wave_argmax2(bidx >= 0 ? big : -1.0, bidx, key, kidx);
+      wave_argmax2(bidx >= 0 ? fabs(big) : -1.0, bidx, key, kidx);
     } else {
       // the 8-wave kernel (128 VGPRs) ran 1.1 % slower with the two maxima
-      key = bidx >= 0 ? (unsigned long long)__double_as_longlong(big) : 0ull;
+      key = bidx >= 0 ? (unsigned long long)__double_as_longlong(fabs(big)) : 0ull;
       kidx = bidx;
       wave_argmax(key, kidx);
     }
@@ -1274,9 +1294,10 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       u0 |= 1ull << icol;
     else
       u1 |= 1ull << (icol - 64);
-    // this lane's columns still open for the next pivot
-    const bool open0 = c0 < mf && !used(c0);
-    const bool open1 = TWO && c1 < mf && !used(c1);
+    // this lane's columns still open for the next pivot (opaque to the
+    // compiler, so that each row tests them with one mask operation)
+    const bool open0 = opaque_flag(c0 < mf && !used(c0));
+    const bool open1 = TWO && opaque_flag(c1 < mf && !used(c1));
     // rows irow and icol before anyone rewrites them
     double pv0 = 0.0, pv1 = 0.0, cv0 = 0.0, cv1 = 0.0;
     if (h0) {
@@ -1324,28 +1345,32 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int ll = l0 + u * GW;
-        const int lr = ll < mf ? ll : w;
+        const int lr = min(ll, mf);  // rows past the last read the dummy row
         dm[u] = a[lr * K + icol];
-        b0[u] = h0 ? a[lr * K + c0] : 0.0;
-        b1[u] = h1 ? a[lr * K + c1] : 0.0;
+        b0[u] = *reinterpret_cast<const double*>(e0 + __umul24(lr, rs0));
+        if constexpr (TWO) b1[u] = *reinterpret_cast<const double*>(e1 + __umul24(lr, rs1));
       }
+      // the batch's rows to eliminate (below mf, not the pivot row), as bits
+      const int nrow = (mf - l0 + GW - 1) / GW;
+      unsigned todo = nrow >= 16 ? 0xFFFFu : (1u << nrow) - 1u;
+      if (icol >= l0 && icol < l0 + 16 * GW && (icol - l0) % GW == 0) todo &= ~(1u << ((icol - l0) / GW));
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
+        if (!((todo >> u) & 1u)) continue;
         const int ll = l0 + u * GW;
-        if (ll >= mf || ll == icol) continue;
         const double dum = dm[u];
         const bool cand = !used(ll);
-        if (h0) {
+        {
           const double base = (c0 == icol) ? 0.0 : b0[u];
           const double nv = fma(-pn0, dum, base);
-          a[ll * K + c0] = nv;
-          if (cand && open0) consider(nv, ll * 128 + c0);
+          *reinterpret_cast<double*>(e0 + __umul24(ll, rs0)) = nv;
+          consider(cand & open0, nv, ll * 128 + c0);
         }
-        if (h1) {
+        if constexpr (TWO) {
           const double base = (c1 == icol) ? 0.0 : b1[u];
           const double nv = fma(-pn1, dum, base);
-          a[ll * K + c1] = nv;
-          if (cand && open1) consider(nv, ll * 128 + c1);
+          *reinterpret_cast<double*>(e1 + __umul24(ll, rs1)) = nv;
+          consider(cand & open1, nv, ll * 128 + c1);
         }
       }
     }
