@@ -1136,6 +1136,14 @@ __device__ __forceinline__ void cand_step(unsigned long long& k, int& i) {
 
 // One step of a wave maximum of a double (lanes outside ROWMASK keep their
 // own value).
+// lane l's value of v, in scalar registers
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
 // b, through a register the compiler cannot see into (keeps a per-lane flag
 // from being merged into the per-row uniform tests that use it)
 __device__ __forceinline__ bool opaque_flag(bool b) {
@@ -1229,12 +1237,15 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
   const bool h0 = c0 < K, h1 = TWO && c1 < K;
   // byte addresses of the elimination's loads and stores, row lr at
   // e0 + lr * rs0: a lane with no column of the matrix (c >= K) loads and
-  // stores a dummy slot in row mf, which the solve never reads, instead of
-  // branching around each row's load and store
+  // stores a dummy slot in row mf (columns 1..K-1), which the solve never
+  // reads, instead of branching around each row's load and store.  Row mf,
+  // column 0 holds 0.0: the pivot column's lane loads it as its base.
   char* const ab = reinterpret_cast<char*>(a);
+  char* const zero = ab + 8 * mf * K;
   const unsigned rs0 = h0 ? 8u * K : 0u, rs1 = h1 ? 8u * K : 0u;
-  char* const e0 = ab + 8 * (h0 ? c0 : mf * K + min(c0 - K, K - 1));
-  char* const e1 = ab + 8 * (h1 ? c1 : mf * K + min(c1 - K, K - 1));
+  char* const e0 = ab + 8 * (h0 ? c0 : mf * K + 1 + min(c0 - K, K - 2));
+  char* const e1 = ab + 8 * (h1 ? c1 : mf * K + 1 + min(c1 - K, K - 2));
+  if (tid == 0) *reinterpret_cast<double*>(zero) = 0.0;  // read after the first barrier
   unsigned long long u0 = 0, u1 = 0;  // pivoted rows / columns
   auto used = [&](int k) -> bool {
     if constexpr (TWO)
@@ -1339,16 +1350,26 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
     }
     big = 0.0;
     bidx = -1;
+    // the elimination's loads: the pivot column's lane reads the zero slot
+    // (its entries become 0 - pn * dum)
+    const bool ic0 = c0 == icol, ic1 = TWO && c1 == icol;
+    const unsigned lrs0 = ic0 ? 0u : rs0, lrs1 = ic1 ? 0u : rs1;
+    char* const le0 = ic0 ? zero : e0;
+    char* const le1 = ic1 ? zero : e1;
     // eliminate this wave's rows
     for (int l0 = w0; l0 < mf; l0 += 16 * GW) {
-      double dm[16], b0[16], b1[16];
+      double b0[16], b1[16];
+      // lane u of each 16 holds row l0 + u*GW's pivot-column entry (rows
+      // past the last read the dummy row), broadcast row by row
+      const int lr16 = min(l0 + (lane & 15) * GW, mf);
+      const double dmv = a[lr16 * K + icol];
+      // rows of the batch still unpivoted, as bits
+      const unsigned cand16 = (unsigned)__builtin_amdgcn_ballot_w64(!used(min(lr16, mf - 1)));
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
-        const int ll = l0 + u * GW;
-        const int lr = min(ll, mf);  // rows past the last read the dummy row
-        dm[u] = a[lr * K + icol];
-        b0[u] = *reinterpret_cast<const double*>(e0 + __umul24(lr, rs0));
-        if constexpr (TWO) b1[u] = *reinterpret_cast<const double*>(e1 + __umul24(lr, rs1));
+        const int lr = min(l0 + u * GW, mf);
+        b0[u] = *reinterpret_cast<const double*>(le0 + __umul24(lr, lrs0));
+        if constexpr (TWO) b1[u] = *reinterpret_cast<const double*>(le1 + __umul24(lr, lrs1));
       }
       // the batch's rows to eliminate (below mf, not the pivot row), as bits
       const int nrow = (mf - l0 + GW - 1) / GW;
@@ -1358,16 +1379,16 @@ __device__ __forceinline__ int gaussj_block(double* a, int mf, int K, double* re
       for (int u = 0; u < 16; ++u) {
         if (!((todo >> u) & 1u)) continue;
         const int ll = l0 + u * GW;
-        const double dum = dm[u];
-        const bool cand = !used(ll);
+        const double dum = readlane_d(dmv, u);
+        const bool cand = (cand16 >> u) & 1u;
         {
-          const double base = (c0 == icol) ? 0.0 : b0[u];
+          const double base = b0[u];
           const double nv = fma(-pn0, dum, base);
           *reinterpret_cast<double*>(e0 + __umul24(ll, rs0)) = nv;
           consider(cand & open0, nv, ll * 128 + c0);
         }
         if constexpr (TWO) {
-          const double base = (c1 == icol) ? 0.0 : b1[u];
+          const double base = b1[u];
           const double nv = fma(-pn1, dum, base);
           *reinterpret_cast<double*>(e1 + __umul24(ll, rs1)) = nv;
           consider(cand & open1, nv, ll * 128 + c1);
