@@ -735,7 +735,7 @@ long long point_row(const Problem& P, const PairInv& v, const double* area, int 
   PointGeo G;
   point_geometry(P, v, pt, G);
   const int nf = P.nfac, C = P.ncoef;
-  double br[4] = {0, 0, 0, 0}, U[4][3] = {}, U0[4][3] = {}, t4[4] = {0, 0, 0, 0}, t5[4] = {0, 0, 0, 0};
+  double U[4][3] = {}, U0[4][3] = {}, t4[4] = {0, 0, 0, 0}, t5[4] = {0, 0, 0, 0};
   std::vector<double> gat(C + 1, 0.0);
   long long nvis = 0;
   for (int f = 0; f < nf; f++) {
@@ -745,24 +745,25 @@ long long point_row(const Problem& P, const PairInv& v, const double* area, int 
     if (mu > TINY && mu0 > TINY) {
       const int s = f & 3;
       const double ar = area[f];
+      /* Lommel-Seeliger + Lambert: S = mu mu0 (cls rd + cl), rd = 1 / (mu
+         + mu0); dS/dmu = mu0 (c2 mu0 + cl) and dS/dmu0 = mu (c2 mu + cl)
+         with c2 = cls rd^2 (Start.cu:947-974, regrouped);
+         the brightness sum ar S = cls t5 + cl t4 is formed after the loop */
       double dnom = mu + mu0;
       double rd = 1.0 / dnom;
-      double mm = mu * mu0;
-      double sc = mm * std::fma(v.cls, rd, v.cl);
-      double w = ar * sc;
-      double q0 = mu0 * rd, q1 = mu * rd;
-      double dsmu = std::fma(v.cls, q0 * q0, v.cl * mu0);
-      double dsmu0 = std::fma(v.cls, q1 * q1, v.cl * mu);
-      double ad = ar * dsmu, ad0 = ar * dsmu0;
+      double am = ar * (mu * mu0);
+      double w = am * std::fma(v.cls, rd, v.cl);
+      double c2 = v.cls * (rd * rd);
+      double ad = ar * (mu0 * std::fma(c2, mu0, v.cl));
+      double ad0 = ar * (mu * std::fma(c2, mu, v.cl));
       U[s][0] = std::fma(ad, n0, U[s][0]);
       U[s][1] = std::fma(ad, n1, U[s][1]);
       U[s][2] = std::fma(ad, n2, U[s][2]);
       U0[s][0] = std::fma(ad0, n0, U0[s][0]);
       U0[s][1] = std::fma(ad0, n1, U0[s][1]);
       U0[s][2] = std::fma(ad0, n2, U0[s][2]);
-      br[s] = br[s] + w;
-      t4[s] = std::fma(ar, mm, t4[s]);
-      t5[s] = std::fma(ar * mm, rd, t5[s]);
+      t4[s] = t4[s] + am;
+      t5[s] = std::fma(am, rd, t5[s]);
       /* shape gather; column 1 (Y_1 == 1) is the size-scale column, used
          only by absolute curves */
       for (int c = 1; c <= C; c++) gat[c] = std::fma(w, P.dsph[(c - 1) * nf + f], gat[c]);
@@ -770,7 +771,8 @@ long long point_row(const Problem& P, const PairInv& v, const double* area, int 
     }
   }
   auto comb = [](const double* x) { return (x[0] + x[1]) + (x[2] + x[3]); };
-  double B = comb(br), T4 = comb(t4), T5 = comb(t5);
+  double T4 = comb(t4), T5 = comb(t5);
+  double B = std::fma(v.cls, T5, v.cl * T4);
   double Uc[3], U0c[3];
   for (int d = 0; d < 3; d++) {
     double a4[4] = {U[0][d], U[1][d], U[2][d], U[3][d]};

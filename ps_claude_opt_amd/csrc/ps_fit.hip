@@ -414,7 +414,7 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
     body_dir(cf, sf, o[RC_VX], o[RC_UY], o[RC_VZ], g);
     body_dir(cf, sf, o[RC_VX0], o[RC_UY0], o[RC_VZ0], g0);
   }
-  double br = 0.0, t4 = 0.0, t5 = 0.0;
+  double t4 = 0.0, t5 = 0.0;
   double U0 = 0.0, U1 = 0.0, U2 = 0.0, V0 = 0.0, V1 = 0.0, V2 = 0.0;
   v4d gat[NBS];
 #pragma unroll
@@ -430,33 +430,34 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   const double* __restrict__ nr = L.nrm + 4 * sl;
   const double* __restrict__ ytab = L.ytab;  // uniform base; step k's rows at ytab + 64 NBS k
   const int ly = (16 * sl + ps) * NBS;       // this lane's offset in a step's rows
-  unsigned nvis = 0;  // the wave's visible (point, facet) pairs (wave-uniform)
+  unsigned nvis = 0;  // this lane's visible (point, facet) pairs
   auto step = [&](double n0, double n1, double n2, double ar, const double* y) {
     const double mu = psm::dot3(g[0], g[1], g[2], n0, n1, n2);
     const double mu0 = psm::dot3(g0[0], g0[1], g0[2], n0, n1, n2);
     const bool vis = (mu > kTiny) && (mu0 > kTiny);
     const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vis);
     if (vmask != 0) {  // a whole step of invisible facets adds exact zeros
-      nvis += (unsigned)__builtin_popcountll(vmask);
       double w = 0.0;
       if (vis) {
+        ++nvis;
+        // S = mu mu0 (cls rd + cl), rd = 1 / (mu + mu0); dS/dmu =
+        // mu0 (c2 mu0 + cl) and dS/dmu0 = mu (c2 mu + cl), c2 = cls rd^2
+        // (Start.cu:947-974, regrouped).  The brightness, sum ar S =
+        // cls t5 + cl t4, is formed per point.
         const double rd = recip(mu + mu0);
-        const double mm = mu * mu0;
-        const double sc = mm * fma(cls, rd, cl);
-        w = ar * sc;
-        const double q0 = mu0 * rd, q1 = mu * rd;
-        const double dsmu = fma(cls, q0 * q0, cl * mu0);
-        const double dsmu0 = fma(cls, q1 * q1, cl * mu);
-        const double ad = ar * dsmu, ad0 = ar * dsmu0;
+        const double am = ar * (mu * mu0);
+        w = am * fma(cls, rd, cl);
+        const double c2 = cls * (rd * rd);
+        const double ad = ar * (mu0 * fma(c2, mu0, cl));
+        const double ad0 = ar * (mu * fma(c2, mu, cl));
         U0 = fma(ad, n0, U0);
         U1 = fma(ad, n1, U1);
         U2 = fma(ad, n2, U2);
         V0 = fma(ad0, n0, V0);
         V1 = fma(ad0, n1, V1);
         V2 = fma(ad0, n2, V2);
-        br = br + w;
-        t4 = fma(ar, mm, t4);
-        t5 = fma(ar * mm, rd, t5);
+        t4 = t4 + am;
+        t5 = fma(am, rd, t5);
       }
 #pragma unroll
       for (int b = 0; b < NBS; ++b) gat[b] = mfma(w, y[b], gat[b]);
@@ -492,9 +493,9 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
 #pragma unroll
     for (int b = 0; b < NBS; ++b) yB[b] = y3[ly + b];
   }
-  br = xsum16_32(br);
   t4 = xsum16_32(t4);
   t5 = xsum16_32(t5);
+  const double br = fma(cls, t5, cl * t4);
   U0 = xsum16_32(U0);
   U1 = xsum16_32(U1);
   U2 = xsum16_32(U2);
@@ -584,7 +585,7 @@ __device__ __forceinline__ unsigned phase1_tile(const PsDevProblem& P, const Lds
   else
     rows_out(std::false_type{});
   wave_sync();
-  return lane == 0 ? nvis : 0u;
+  return nvis;
 }
 
 // ---------------------------------------------------------------------------

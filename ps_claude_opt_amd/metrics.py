@@ -9,7 +9,9 @@ data points, F facets, C shape coefficients, M free parameters, K = M + 1,
 and V visible (point, facet) pairs:
 
   visibility           10 N F           two 3-term dot products
-  visible facets       V (37 + 2 (C-1)) scattering law, derivative sums,
+  visible facets       V (32 + 2 (C-1)) scattering law with its derivative
+                                        sums (regrouped: 1 / (mu + mu0), its
+                                        square and the products once), the
                                         shape-Jacobian gather
   per point            N (91 + (C-1) + 3M + 5 + K(K+1) + K)
                                         geometry, derivative columns, scaling,
@@ -29,7 +31,7 @@ from __future__ import annotations
 def flops_per_eval(N: int, F: int, C: int, M: int, V: float) -> float:
     K = M + 1
     per_point = 91 + (C - 1) + 3 * M + 5 + K * (K + 1) + K
-    return (10.0 * N * F + V * (37 + 2 * (C - 1)) + N * per_point + N * (M + 1)
+    return (10.0 * N * F + V * (32 + 2 * (C - 1)) + N * per_point + N * (M + 1)
             + F * (2 * C + 1) + 3 * F * (2 * C + 1))
 
 
