@@ -134,7 +134,8 @@ def main() -> int:
     # finish their pairs while the others have no work left, costs about 4 %
     # at 5,000 pair-fits per launch and 0.2 % here.
     batch = max(1, min(n_max, args.freqs_per_step if args.freqs_per_step > 0 else -(-n_max // steps)))
-    tune = autotune(scan, np.zeros(1, np.int32), np.array([n_max]), conw_r, args.no_tune)
+    heavy = scan.info["ndata"] * scan.info["nfac"] > 4_000_000  # lmax 8: 20,800 points x 512 facets
+    tune = autotune(scan, np.zeros(1, np.int32), np.array([n_max]), conw_r, args.no_tune, k=3072 if heavy else 12288)
     # warmup on the first batch (untimed)
     for _ in range(args.warmup):
         scan.start(conw_r, 1, batch)
@@ -290,10 +291,11 @@ def bench_warm(scan, conw_r: float, n_max: int, plain_best, n_iter: int, refit: 
     return out
 
 
-def autotune(scan, probs, n_max, conw, skip: bool) -> dict:
+def autotune(scan, probs, n_max, conw, skip: bool, k: int = 12288) -> dict:
     """Picks the launch configuration (wavefronts per pair, workgroups per
-    CU, facet-pass claim) on a seeded sample of 12,288 of the workload's
-    pairs, before the timed region.  The configurations give bit-identical
+    CU, facet-pass claim) on a seeded sample of k of the workload's pairs
+    (12,288, or 3,072 for the lmax-8 workload, whose pairs cost 20 times
+    more), before the timed region.  The configurations give bit-identical
     results."""
     import numpy as np
 
@@ -301,9 +303,9 @@ def autotune(scan, probs, n_max, conw, skip: bool) -> dict:
         return dict(scan.get_launch(), tuned=False)
     rng = np.random.default_rng(777)
     # a whole number of rounds of pairs for every launch shape on 256 CUs (2,
-    # 3, 4, 6, 8 or 12 resident workgroups per CU all divide 12,288), so that
-    # no shape gains or loses from the size of its last round
-    k = 12288
+    # 3, 4, 6, 8 or 12 resident workgroups per CU all divide 12,288; 2, 3, 4
+    # or 12 divide 3,072), so that no shape gains or loses from the size of
+    # its last round
     pi = rng.integers(0, len(probs), k)
     p = np.asarray(probs)[pi]
     nl = np.array([rng.integers(1, int(np.asarray(n_max)[j]) + 1) for j in pi])
