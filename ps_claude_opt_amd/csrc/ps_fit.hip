@@ -1224,9 +1224,10 @@ __device__ __forceinline__ void wave_argmax(unsigned long long& k, int& i) {
 // irow and icol before the second; after it, the waves that own those rows
 // store the swapped rows (icol: the scaled pivot row, irow: old row icol)
 // and every wave eliminates its rows, loading up to 16 of them per batch.
-// Returns 0, or 2 for a zero pivot (or nothing selectable); the solution is
-// left in column mf.  Every thread of the workgroup must call it.
-// TWO: columns beyond 64 exist (K > 64).
+// a holds K * K doubles: row mf is the solve's own scratch (a zero slot and
+// the dummy slots below).  Returns 0, or 2 for a zero pivot (or nothing
+// selectable); the solution is left in column mf.  Every thread of the
+// workgroup must call it.  TWO: columns beyond 64 exist (K > 64).
 constexpr int GW = NW > 4 ? 4 : NW;  // waves that solve
 
 template <bool TWO>
