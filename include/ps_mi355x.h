@@ -346,8 +346,9 @@ int ps_eval(ps_ctx* ctx, const double* a, double conw_r2_rows, double* alpha,
             double* beta, double* chisq, double* area, double* jrows);
 
 /* The damped Gauss-Jordan solve of mrqmin step 1 (Start.cu:2838-3021) on
- * the device: x = (alpha with diagonal*(1+lambda))^-1 beta.  Returns the
- * reference error code (0, 1, 2) in *err. */
+ * the device: x = (alpha with diagonal*(1+lambda))^-1 beta, by the fit
+ * kernels' solver of the wave count ps_set_launch configured (4 by
+ * default).  Returns the reference error code (0, 1, 2) in *err. */
 int ps_gaussj(ps_ctx* ctx, int mfit, const double* alpha, const double* beta,
               double lambda, double* x, int* err);
 

@@ -109,16 +109,17 @@ struct FitVariant {
   int (*blocks_per_cu)(int, int, size_t);
   hipError_t (*launch)(const PsFitArgs&, int, hipStream_t);
   hipError_t (*phase_cycles)(unsigned long long*, bool);
+  hipError_t (*gaussj)(int, const double*, const double*, double, double*, int*, hipStream_t);
 };
 const FitVariant kVariants[4] = {
     {1, psw1::ps_fit_lds_bytes, psw1::ps_set_fit_attributes, psw1::ps_fit_blocks_per_cu, psw1::ps_launch_fit,
-     psw1::ps_phase_cycles},
+     psw1::ps_phase_cycles, psw1::ps_launch_gaussj},
     {2, psw2::ps_fit_lds_bytes, psw2::ps_set_fit_attributes, psw2::ps_fit_blocks_per_cu, psw2::ps_launch_fit,
-     psw2::ps_phase_cycles},
+     psw2::ps_phase_cycles, psw2::ps_launch_gaussj},
     {4, psw4::ps_fit_lds_bytes, psw4::ps_set_fit_attributes, psw4::ps_fit_blocks_per_cu, psw4::ps_launch_fit,
-     psw4::ps_phase_cycles},
+     psw4::ps_phase_cycles, psw4::ps_launch_gaussj},
     {8, psw8::ps_fit_lds_bytes, psw8::ps_set_fit_attributes, psw8::ps_fit_blocks_per_cu, psw8::ps_launch_fit,
-     psw8::ps_phase_cycles},
+     psw8::ps_phase_cycles, psw8::ps_launch_gaussj},
 };
 int variant_index(int waves) { return waves == 1 ? 0 : waves == 2 ? 1 : waves == 8 ? 3 : 2; }
 // The variant the next launch uses: the configured one if it can be
@@ -1249,7 +1250,8 @@ int ps_gaussj(ps_ctx* c, int mfit, const double* alpha, const double* beta, doub
   PS_HIP(c, bufs.alloc(&de, 1));
   PS_HIP(c, hipMemcpy(da, alpha, (size_t)mfit * mfit * sizeof(double), hipMemcpyHostToDevice));
   PS_HIP(c, hipMemcpy(db, beta, mfit * sizeof(double), hipMemcpyHostToDevice));
-  PS_HIP(c, psw4::ps_launch_gaussj(mfit, da, db, lambda, dx, de, c->stream));
+  // the Gauss-Jordan of the configured wave count (ps_set_launch)
+  PS_HIP(c, kVariants[variant_index(c->waves)].gaussj(mfit, da, db, lambda, dx, de, c->stream));
   PS_HIP(c, hipStreamSynchronize(c->stream));
   PS_HIP(c, hipMemcpy(x, dx, mfit * sizeof(double), hipMemcpyDeviceToHost));
   PS_HIP(c, hipMemcpy(err, de, sizeof(int), hipMemcpyDeviceToHost));

@@ -82,7 +82,7 @@ def test_jacobian_rows_bit_identical(scan, inputs, name):
     assert np.array_equal(J0, J1, equal_nan=True)
 
 
-def test_gaussj_bit_identical(scan, inputs):
+def _gaussj_cases(scan, inputs):
     rng = np.random.default_rng(11)
     cases = []
     for n in (8, 33, 54, 86):
@@ -100,11 +100,32 @@ def test_gaussj_bit_identical(scan, inputs):
     sing = np.eye(6)
     sing[3, 3] = 0.0
     cases.append((sing, np.ones(6), 0.0))
-    for A_, b_, lam in cases:
+    return cases
+
+
+def test_gaussj_bit_identical(scan, inputs):
+    for A_, b_, lam in _gaussj_cases(scan, inputs):
         x0, e0 = po.gaussj(A_, b_, lam)
         x1, e1 = scan.gaussj(A_, b_, lam)
         assert e0 == e1
         assert np.array_equal(x0, x1, equal_nan=True), (len(b_), np.abs(x0 - x1).max())
+
+
+def test_gaussj_every_wave_count(scan, inputs):
+    """The solver of each kernel variant (1, 2, 4 or 8 waves per workgroup,
+    of which 1, 2, 4 and 4 own rows) agrees with the oracle bit for bit on
+    the same systems."""
+    before = scan.get_launch()
+    try:
+        for w in (1, 2, 8, 4):
+            scan.set_launch(w, 0)
+            for A_, b_, lam in _gaussj_cases(scan, inputs):
+                x0, e0 = po.gaussj(A_, b_, lam)
+                x1, e1 = scan.gaussj(A_, b_, lam)
+                assert e0 == e1, (w, len(b_))
+                assert np.array_equal(x0, x1, equal_nan=True), (w, len(b_), np.abs(x0 - x1).max())
+    finally:
+        scan.set_launch(before["waves_per_pair"], before["wg_per_cu"])
 
 
 def _same(a, b):
