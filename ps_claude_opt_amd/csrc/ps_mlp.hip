@@ -27,14 +27,21 @@ __device__ __forceinline__ short to_bf16(float x) {
   return (short)(u >> 16);
 }
 
+// GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi)
+__device__ __forceinline__ float gelu(float x) {
+  const float s = 0.7978845608028654f;
+  return 0.5f * x * (1.f + tanhf(s * (x + 0.044715f * x * x * x)));
+}
+
 // C[m][n] = sum_k A(m,k) B(k,n) (+ bias[n]); A(m,k) = TA ? A[k*lda+m] :
-// A[m*lda+k], B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n].  One wave per 16 x 16
-// output block: lane l holds row l%16 of A and column l%16 of B over the
-// k-values KP*(l/16) .. +KP-1 of each step, and D rows 4*(l/16)+r.
+// A[m*lda+k], B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n]; with H, also H[m][n] =
+// GELU(C[m][n]).  One wave per 16 x 16 output block: lane l holds row l%16
+// of A and column l%16 of B over the k-values KP*(l/16) .. +KP-1 of each
+// step, and D rows 4*(l/16)+r.
 template <bool BF16, bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
                                                    const float* __restrict__ B, int ldb, const float* __restrict__ bias,
-                                                   float* __restrict__ C, int ldc) {
+                                                   float* __restrict__ C, int ldc, float* __restrict__ H) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int m0 = blockIdx.y * TILE + (w >> 1) * 16, n0 = blockIdx.x * TILE + (w & 1) * 16;
   const int r16 = lane & 15, g = lane >> 4;
@@ -68,15 +75,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const fl
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int cm = m0 + 4 * g + r;
-      if (cm < M) C[(size_t)cm * ldc + cn] = acc[r] + bv;
+      if (cm < M) {
+        const float zv = acc[r] + bv;
+        C[(size_t)cm * ldc + cn] = zv;
+        if (H) H[(size_t)cm * ldc + cn] = gelu(zv);
+      }
     }
   }
-}
-
-// GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi)
-__device__ __forceinline__ float gelu(float x) {
-  const float s = 0.7978845608028654f;
-  return 0.5f * x * (1.f + tanhf(s * (x + 0.044715f * x * x * x)));
 }
 
 __device__ __forceinline__ float gelu_grad(float x) {
@@ -84,11 +89,6 @@ __device__ __forceinline__ float gelu_grad(float x) {
   const float u = s * (x + 0.044715f * x * x * x);
   const float t = tanhf(u);
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * s * (1.f + 3.f * 0.044715f * x * x);
-}
-
-__global__ void gelu_kernel(int n, const float* __restrict__ z, float* __restrict__ h) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) h[i] = gelu(z[i]);
 }
 
 __global__ void gelu_bwd_kernel(int n, const float* __restrict__ dh, const float* __restrict__ z,
@@ -151,12 +151,12 @@ inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 }  // namespace
 
 hipError_t psm_gemm(bool bf16, bool ta, bool tb, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
-                    const float* bias, float* C, int ldc, hipStream_t s) {
+                    const float* bias, float* C, int ldc, hipStream_t s, float* H) {
   const dim3 grid((N + TILE - 1) / TILE, (M + TILE - 1) / TILE);
 #define PS_G(BF, TA_, TB_)                                                                                     \
   if (bf16 == BF && ta == TA_ && tb == TB_) {                                                                 \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_kernel<BF, TA_, TB_>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, \
-                       bias, C, ldc);                                                                          \
+                       bias, C, ldc, H);                                                                       \
     return hipGetLastError();                                                                                  \
   }
   PS_G(true, false, false)
@@ -167,11 +167,6 @@ hipError_t psm_gemm(bool bf16, bool ta, bool tb, int M, int N, int K, const floa
   PS_G(false, false, true)
 #undef PS_G
   return hipErrorInvalidValue;
-}
-
-hipError_t psm_gelu(int n, const float* z, float* h, hipStream_t s) {
-  hipLaunchKernelGGL(gelu_kernel, grid1(n), dim3(256), 0, s, n, z, h);
-  return hipGetLastError();
 }
 
 hipError_t psm_gelu_bwd(int n, const float* dh, const float* z, float* dz, hipStream_t s) {

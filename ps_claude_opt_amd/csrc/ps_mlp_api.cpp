@@ -78,12 +78,12 @@ int forward(ps_mlp* net, int rows, bool bf16) {
   hipStream_t s = net->ctx->stream;
   const float* in = net->x;
   for (int l = 0; l <= net->n_layers; ++l) {
+    // hidden layers: pre-activation z (kept for the backward pass) and its
+    // GELU h from the same launch
+    float* h = l < net->n_layers ? net->h[l] : nullptr;
     MLP_HIP(psm_gemm(bf16, false, false, rows, net->kout[l], net->kin[l], in, net->kin[l], net->w + net->w_off[l],
-                     net->kout[l], net->w + net->b_off[l], net->z[l], net->kout[l], s));
-    if (l < net->n_layers) {
-      MLP_HIP(psm_gelu(rows * net->kout[l], net->z[l], net->h[l], s));
-      in = net->h[l];
-    }
+                     net->kout[l], net->w + net->b_off[l], net->z[l], net->kout[l], s, h));
+    if (h) in = h;
   }
   return PS_OK;
 }
