@@ -210,7 +210,9 @@ def make_training_set(scan, n_pairs_per_problem: int, conw_r: Sequence[float], s
     feats = scan.warm_features(prob, nl, ml)
     a_init = scan.initial_params(prob, nl, ml)
     Cn = scan.info["ncoef"]
-    tg = (targets_from_params(a_final, Cn) - targets_from_params(a_init, Cn)).astype(np.float32)
+    # runaway fits overflow the FP32 cast to inf; the finiteness test below drops them
+    with np.errstate(over="ignore"):
+        tg = (targets_from_params(a_final, Cn) - targets_from_params(a_init, Cn)).astype(np.float32)
     ok = np.isfinite(tg).all(1) & np.isfinite(res["dev"]) & (res["dev"] < 1e30)
     # fits that ran away (phase-function parameters in the 1e20s and beyond
     # with a finite rms) are no examples of where a fit goes: keep rows whose
