@@ -91,8 +91,8 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * s * (1.f + 3.f * 0.044715f * x * x);
 }
 
-__global__ void gelu_bwd_kernel(int n, const float* __restrict__ dh, const float* __restrict__ z,
-                                float* __restrict__ dz) {
+// dz may be dh (the backward pass updates the gradient in place)
+__global__ void gelu_bwd_kernel(int n, const float* dh, const float* __restrict__ z, float* dz) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dz[i] = dh[i] * gelu_grad(z[i]);
 }
